@@ -1,0 +1,148 @@
+// Host-side helpers of the run functions that physisc-hip generates: launch
+// planning for 2.5D run kernels, argument checks and the run timer that backs
+// --physis-trace (reference: translator/reference_runtime_builder.cc:896-940
+// emits the equivalent stopwatch code inline).
+#ifndef PHYSIS_HIP_RUN_H_
+#define PHYSIS_HIP_RUN_H_
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+
+#include "physis/physis_hip.h"
+#include "physis/hip/stencil.h"
+
+namespace physis_hip {
+
+// Resident blocks of `kernel` per compute unit at `threads` per block.
+inline int BlocksPerCU(const void *kernel, int threads) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0) !=
+          hipSuccess ||
+      n < 1) {
+    (void)hipGetLastError();
+    n = 1;
+  }
+  return n;
+}
+
+// Tiles a domain into TX x TY column tiles and picks the number of z chunks.
+// Each block costs (chunk + 2) plane steps (two prologue planes), blocks run
+// in waves of num_cus * per_cu; the chunk count minimising
+// waves * (chunk + 2) is chosen so that the last wave is not left mostly idle.
+inline RunShape PlanRun(const __PSDomain &dom, int tx, int ty, int per_cu) {
+  RunShape s = {0, 0, 0, 0, 0, 0};
+  const int nx = dom.local_max[0] - dom.local_min[0];
+  const int ny = dom.local_max[1] - dom.local_min[1];
+  const int nz = dom.local_max[2] - dom.local_min[2];
+  if (nx <= 0 || ny <= 0 || nz <= 0) return s;
+  // x tiles are anchored at x = 0 so that vector loads stay aligned.
+  const int xf = dom.local_min[0] >= 0 ? dom.local_min[0] / tx
+                                       : -((-dom.local_min[0] + tx - 1) / tx);
+  const int xl = (dom.local_max[0] - 1) >= 0
+                     ? (dom.local_max[0] - 1) / tx
+                     : -((-(dom.local_max[0] - 1) + tx - 1) / tx);
+  s.x_first_tile = xf;
+  s.tiles_x = xl - xf + 1;
+  s.tiles_y = (ny + ty - 1) / ty;
+  const long xy = (long)s.tiles_x * s.tiles_y;
+  const long resident = (long)__PSHipNumCUs() * per_cu;
+  long best_cost = -1;
+  int best_tz = 1;
+  for (int tz = 1; tz <= nz; ++tz) {
+    const int chunk = (nz + tz - 1) / tz;
+    if ((long)(tz - 1) * chunk >= nz) continue;  // same chunk, fewer blocks
+    const long blocks = xy * tz;
+    if (blocks > (1L << 30)) break;
+    const long waves = (blocks + resident - 1) / resident;
+    const long cost = waves * (chunk + 2);
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best_tz = tz;
+    }
+  }
+  s.tiles_z = best_tz;
+  s.zchunk = (nz + best_tz - 1) / best_tz;
+  s.blocks = (int)(xy * best_tz);
+  return s;
+}
+
+// A 3-D grid argument must be live, of the stencil's element type, and small
+// enough for 32-bit PSIndex offsets (the reference's default index type).
+inline void CheckGrid3D(const __PSGrid *g, int type) {
+  if (!g || !g->p) {
+    std::fprintf(stderr, "[physis-hip] stencil map on a freed or NULL grid\n");
+    std::exit(EXIT_FAILURE);
+  }
+  if (g->type != type) {
+    std::fprintf(stderr,
+                 "[physis-hip] grid element type %d does not match the "
+                 "stencil's type %d\n", g->type, type);
+    std::exit(EXIT_FAILURE);
+  }
+  if (g->num_elms >= (int64_t)PSINDEX_MAX) {
+    std::fprintf(stderr,
+                 "[physis-hip] grid of %lld elements exceeds 32-bit PSIndex\n",
+                 (long long)g->num_elms);
+    std::exit(EXIT_FAILURE);
+  }
+}
+
+// The domain of a map must lie inside every grid the map emits to; the run
+// kernel stores at every domain point.
+inline void CheckEmitDomain(const __PSDomain &dom, const __PSGrid *g) {
+  for (int d = 0; d < g->num_dims; ++d) {
+    if (dom.local_max[d] <= dom.local_min[d]) return;  // empty domain
+  }
+  for (int d = 0; d < g->num_dims; ++d) {
+    if (dom.local_min[d] < 0 || dom.local_max[d] > g->dim[d]) {
+      std::fprintf(stderr,
+                   "[physis-hip] domain [%d,%d) on axis %d exceeds the "
+                   "emitted grid's extent %d\n",
+                   dom.local_min[d], dom.local_max[d], d, g->dim[d]);
+      std::exit(EXIT_FAILURE);
+    }
+  }
+}
+
+// Brackets a run function: times it with HIP events when tracing is on and
+// always waits for the stream at the end, as the reference run function
+// does after its launch loop (cuda_runtime_builder.cc:1490-1507).
+class RunTimer {
+ public:
+  explicit RunTimer(hipStream_t stream) : stream_(stream) {
+    if (__ps_trace) {
+      (void)hipEventCreate(&start_);
+      (void)hipEventCreate(&stop_);
+      (void)hipEventRecord(start_, stream_);
+    }
+  }
+  float Stop() {
+    float ms = 0.0f;
+    if (__ps_trace) {
+      (void)hipEventRecord(stop_, stream_);
+    }
+    hipError_t e = hipStreamSynchronize(stream_);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) {
+      std::fprintf(stderr, "[physis-hip] stencil run failed: %s\n",
+                   hipGetErrorString(e));
+      std::exit(EXIT_FAILURE);
+    }
+    if (__ps_trace) {
+      (void)hipEventElapsedTime(&ms, start_, stop_);
+      (void)hipEventDestroy(start_);
+      (void)hipEventDestroy(stop_);
+    }
+    return ms;
+  }
+
+ private:
+  hipStream_t stream_;
+  hipEvent_t start_ = nullptr, stop_ = nullptr;
+};
+
+}  // namespace physis_hip
+
+#endif  // PHYSIS_HIP_RUN_H_

@@ -1,0 +1,100 @@
+"""In-tree build of libphysis_rt_hip and of the translated Physis programs.
+
+Everything is compiled for gfx950 with hipcc and placed in physis_amd/lib/,
+next to the Python host mirror that loads it.  Called by
+__graft_entry__.build() and usable as `python -m physis_amd.build`.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+INCLUDE = os.path.join(ROOT, "include")
+LIB = os.path.join(PKG, "lib")
+CSRC = os.path.join(PKG, "csrc")
+PROGRAMS = os.path.join(PKG, "programs")
+
+ARCH = "gfx950"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# -ffp-contract=off keeps a*b+c as two roundings, like the REFERENCE target
+# compiled by gcc for x86-64 (no FMA), which is what bit-exact parity needs.
+# Packed FP32 instructions are disabled (toolchain notes).
+HIP_FLAGS = [
+    "-x", "hip", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    "-ffp-contract=off", "-Wall", "-Wno-unused-function",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    f"-I{INCLUDE}",
+]
+
+RUNTIME_SOURCES = [
+    os.path.join(CSRC, "runtime", "libphysis_rt_hip.cc"),
+    os.path.join(CSRC, "runtime", "reduce_hip.hip"),
+    os.path.join(CSRC, "runtime", "buffer_hip.hip"),
+]
+
+# Translated programs built into their own shared libraries.
+PROGRAM_NAMES = ["diffusion7"]
+
+
+def _run(cmd: list[str]) -> None:
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                          text=True)
+    out = "\n".join(line for line in proc.stdout.splitlines()
+                    if "packed-fp32-ops" not in line
+                    and "argument unused during compilation" not in line)
+    if proc.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{out}")
+    if out.strip():
+        print(out)
+
+
+def _compile_object(src: str, obj: str) -> None:
+    _run([HIPCC, *HIP_FLAGS, "-c", src, "-o", obj])
+
+
+def build_runtime(objdir: str) -> str:
+    objs = []
+    jobs = []
+    for src in RUNTIME_SOURCES:
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        objs.append(obj)
+        jobs.append((src, obj))
+    with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        list(ex.map(lambda j: _compile_object(*j), jobs))
+    out = os.path.join(LIB, "libphysis_rt_hip.so")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs,
+          "-o", out])
+    return out
+
+
+def build_program(name: str) -> str:
+    src = os.path.join(PROGRAMS, f"{name}.hip.cpp")
+    out = os.path.join(LIB, f"libphysis_prog_{name}.so")
+    _run([HIPCC, *HIP_FLAGS, "-shared", src, "-o", out, f"-L{LIB}",
+          "-lphysis_rt_hip", "-Wl,-rpath,$ORIGIN"])
+    return out
+
+
+def build_all(verbose: bool = True) -> list[str]:
+    os.makedirs(LIB, exist_ok=True)
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    built = [build_runtime(objdir)]
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        built += list(ex.map(build_program, PROGRAM_NAMES))
+    shutil.rmtree(objdir, ignore_errors=True)
+    if verbose:
+        for b in built:
+            print("built", os.path.relpath(b, ROOT))
+    return built
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

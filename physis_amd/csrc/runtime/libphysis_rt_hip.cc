@@ -1,0 +1,304 @@
+// libphysis_rt_hip: the Physis runtime for one MI355X.
+//
+// Replaces the CUDA target runtime (reference runtime/libphysis_rt_cuda.cc)
+// entry point for entry point; see include/physis/physis_hip.h for the
+// contract of each function.
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+#include "runtime_hip.h"
+
+FILE *__ps_trace = nullptr;
+
+namespace physis_hip {
+
+Runtime &rt() {
+  static Runtime r;
+  return r;
+}
+
+void EnsureInit() {
+  Runtime &r = rt();
+  if (r.initialized) return;
+  int ndev = 0;
+  PS_HIP_CHECK(hipGetDeviceCount(&ndev));
+  if (ndev < 1) PS_FATAL("no HIP device found");
+  int dev = 0;
+  if (const char *s = std::getenv("PHYSIS_HIP_DEVICE")) dev = std::atoi(s);
+  if (dev < 0 || dev >= ndev) PS_FATAL("PHYSIS_HIP_DEVICE=%d out of range", dev);
+  PS_HIP_CHECK(hipSetDevice(dev));
+  hipDeviceProp_t prop;
+  PS_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+  r.device = dev;
+  r.num_cus = prop.multiProcessorCount;
+  r.stream = nullptr;
+  r.initialized = true;
+}
+
+void *Scratch(size_t bytes) {
+  Runtime &r = rt();
+  EnsureInit();
+  if (bytes > r.scratch_bytes) {
+    if (r.scratch) {
+      PS_HIP_CHECK(hipStreamSynchronize(r.stream));
+      PS_HIP_CHECK(hipFree(r.scratch));
+    }
+    size_t cap = bytes < (1u << 20) ? (1u << 20) : bytes;
+    PS_HIP_CHECK(hipMalloc(&r.scratch, cap));
+    r.scratch_bytes = cap;
+  }
+  return r.scratch;
+}
+
+void *HostResultSlot() {
+  Runtime &r = rt();
+  EnsureInit();
+  if (!r.host_result) PS_HIP_CHECK(hipHostMalloc(&r.host_result, 256));
+  return r.host_result;
+}
+
+namespace {
+
+// Removes "--physis-trace" from argv and turns tracing on, as the reference
+// runtime does (runtime/runtime.h:20-30, runtime_common.cc:14-35).
+void ParseTraceOption(int *argc, char ***argv) {
+  if (!argc || !argv || !*argv) return;
+  int out = 0;
+  for (int i = 0; i < *argc; ++i) {
+    const char *a = (*argv)[i];
+    if (a && (std::strcmp(a, "--physis-trace") == 0 ||
+              std::strcmp(a, "-physis-trace") == 0)) {
+      __ps_trace = stderr;
+      continue;
+    }
+    (*argv)[out++] = (*argv)[i];
+  }
+  *argc = out;
+}
+
+PSType TypeFromSize(const __PSGridTypeInfo *ti) {
+  if (ti) return ti->type;
+  return PS_FLOAT;
+}
+
+void CopyIndices(va_list vl, int nd, PSIndex *idx) {
+  for (int i = 0; i < nd; ++i) idx[i] = va_arg(vl, PSIndex);
+}
+
+int64_t LinearOffset(const __PSGrid *g, const PSIndex *idx) {
+  int64_t off = 0, base = 1;
+  for (int i = 0; i < g->num_dims; ++i) {
+    if (idx[i] < 0 || idx[i] >= g->dim[i])
+      PS_FATAL("grid index %d out of range [0,%d) on axis %d", idx[i],
+               g->dim[i], i);
+    off += (int64_t)idx[i] * base;
+    base *= g->dim[i];
+  }
+  return off;
+}
+
+}  // namespace
+}  // namespace physis_hip
+
+using namespace physis_hip;
+
+extern "C" {
+
+void PSInit(int *argc, char ***argv, int grid_num_dims, ...) {
+  (void)grid_num_dims;
+  ParseTraceOption(argc, argv);
+  EnsureInit();
+}
+
+void PSFinalize(void) {
+  Runtime &r = rt();
+  if (!r.initialized) return;
+  PS_HIP_CHECK(hipDeviceSynchronize());
+  if (r.scratch) PS_HIP_CHECK(hipFree(r.scratch));
+  if (r.host_result) PS_HIP_CHECK(hipHostFree(r.host_result));
+  r.scratch = nullptr;
+  r.scratch_bytes = 0;
+  r.host_result = nullptr;
+}
+
+int __PSGridGetID(__PSGrid *g) {
+  (void)g;
+  return 0;
+}
+
+__PSGrid *__PSGridNew(__PSGridTypeInfo *type_info, int num_dims,
+                      PSVectorInt dim, __PSGrid_devNewFunc func) {
+  EnsureInit();
+  if (num_dims < 1 || num_dims > PS_MAX_DIM)
+    PS_FATAL("unsupported grid dimensionality %d", num_dims);
+  __PSGrid *g = (__PSGrid *)std::calloc(1, sizeof(__PSGrid));
+  if (!g) return INVALID_GRID;
+  g->elm_size = type_info ? type_info->size : (int)sizeof(float);
+  g->num_dims = num_dims;
+  g->type = TypeFromSize(type_info);
+  g->num_elms = 1;
+  for (int i = 0; i < PS_MAX_DIM; ++i) g->dim[i] = 1;
+  for (int i = 0; i < num_dims; ++i) {
+    if (dim[i] < 0) PS_FATAL("negative grid extent %d", dim[i]);
+    g->dim[i] = dim[i];
+    g->num_elms *= dim[i];
+  }
+  if (func) {
+    g->dev = (__PSGrid_dev *)func(num_dims, dim);
+    g->p = nullptr;
+    return g;
+  }
+  size_t bytes = (size_t)g->num_elms * (size_t)g->elm_size;
+  void *buf = nullptr;
+  if (bytes > 0) {
+    hipError_t e = hipMalloc(&buf, bytes);
+    if (e != hipSuccess || !buf) {
+      (void)hipGetLastError();  // clear the sticky allocation error
+      std::fprintf(stderr,
+                   "[physis-hip] __PSGridNew: cannot allocate %zu bytes: %s\n",
+                   bytes, hipGetErrorString(e));
+      std::free(g);
+      return INVALID_GRID;
+    }
+    // Grids start zero-filled like the reference (cudaMemset / calloc).
+    PS_HIP_CHECK(hipMemsetAsync(buf, 0, bytes, rt().stream));
+  }
+  g->p = buf;
+  g->dev = (__PSGrid_dev *)std::calloc(1, sizeof(__PSGrid_dev));
+  for (int i = 0; i < 3; ++i) g->dev->dim[i] = g->dim[i];
+  g->dev->p = buf;
+  return g;
+}
+
+void __PSGridFree(void *gv, __PSGrid_devFreeFunc func) {
+  __PSGrid *g = (__PSGrid *)gv;
+  if (!g) return;
+  if (g->dev) {
+    if (func) {
+      func(g->dev);
+    } else {
+      if (g->p) {
+        PS_HIP_CHECK(hipStreamSynchronize(rt().stream));
+        PS_HIP_CHECK(hipFree(g->p));
+      }
+      std::free(g->dev);
+    }
+    g->dev = nullptr;
+  }
+  g->p = nullptr;
+  std::free(g);
+}
+
+void __PSGridCopyin(void *gv, const void *src_array,
+                    __PSGrid_devCopyinFunc func) {
+  __PSGrid *g = (__PSGrid *)gv;
+  if (func) {
+    func(g->dev, src_array, (size_t)g->num_elms);
+    return;
+  }
+  size_t bytes = (size_t)g->num_elms * (size_t)g->elm_size;
+  if (bytes == 0) return;
+  PS_HIP_CHECK(hipMemcpyAsync(g->p, src_array, bytes, hipMemcpyHostToDevice,
+                              rt().stream));
+  PS_HIP_CHECK(hipStreamSynchronize(rt().stream));
+}
+
+void __PSGridCopyout(void *gv, void *dst_array,
+                     __PSGrid_devCopyoutFunc func) {
+  __PSGrid *g = (__PSGrid *)gv;
+  if (func) {
+    func(g->dev, dst_array, (size_t)g->num_elms);
+    return;
+  }
+  size_t bytes = (size_t)g->num_elms * (size_t)g->elm_size;
+  if (bytes == 0) return;
+  PS_HIP_CHECK(hipMemcpyAsync(dst_array, g->p, bytes, hipMemcpyDeviceToHost,
+                              rt().stream));
+  PS_HIP_CHECK(hipStreamSynchronize(rt().stream));
+}
+
+void __PSGridSwap(__PSGrid *g) { (void)g; }
+
+void __PSGridSet(__PSGrid *g, void *buf, ...) {
+  PSIndex idx[PS_MAX_DIM] = {0, 0, 0};
+  va_list vl;
+  va_start(vl, buf);
+  CopyIndices(vl, g->num_dims, idx);
+  va_end(vl);
+  int64_t off = LinearOffset(g, idx) * g->elm_size;
+  PS_HIP_CHECK(hipMemcpyAsync((char *)g->p + off, buf, g->elm_size,
+                              hipMemcpyHostToDevice, rt().stream));
+  PS_HIP_CHECK(hipStreamSynchronize(rt().stream));
+}
+
+void __PSGridGet(__PSGrid *g, void *buf, ...) {
+  PSIndex idx[PS_MAX_DIM] = {0, 0, 0};
+  va_list vl;
+  va_start(vl, buf);
+  CopyIndices(vl, g->num_dims, idx);
+  va_end(vl);
+  int64_t off = LinearOffset(g, idx) * g->elm_size;
+  PS_HIP_CHECK(hipMemcpyAsync(buf, (const char *)g->p + off, g->elm_size,
+                              hipMemcpyDeviceToHost, rt().stream));
+  PS_HIP_CHECK(hipStreamSynchronize(rt().stream));
+}
+
+PSDomain1D PSDomain1DNew(PSIndex minx, PSIndex maxx) {
+  PSDomain1D d = {{minx, 0, 0}, {maxx, 0, 0}, {minx, 0, 0}, {maxx, 0, 0}};
+  return d;
+}
+
+PSDomain2D PSDomain2DNew(PSIndex minx, PSIndex maxx, PSIndex miny,
+                         PSIndex maxy) {
+  PSDomain2D d = {{minx, miny, 0}, {maxx, maxy, 0},
+                  {minx, miny, 0}, {maxx, maxy, 0}};
+  return d;
+}
+
+PSDomain3D PSDomain3DNew(PSIndex minx, PSIndex maxx, PSIndex miny,
+                         PSIndex maxy, PSIndex minz, PSIndex maxz) {
+  PSDomain3D d = {{minx, miny, minz}, {maxx, maxy, maxz},
+                  {minx, miny, minz}, {maxx, maxy, maxz}};
+  return d;
+}
+
+void __PSCheckHipError(const char *message) {
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(rt().stream);
+  if (e != hipSuccess) {
+    std::fprintf(stderr, "ERROR: %s: %s\n", message, hipGetErrorString(e));
+    PSAbort(1);
+  }
+}
+
+void __PSReduceGridFloat(void *buf, enum PSReduceOp op, __PSGrid *g) {
+  ReduceGrid(buf, op, g, PS_FLOAT);
+}
+void __PSReduceGridDouble(void *buf, enum PSReduceOp op, __PSGrid *g) {
+  ReduceGrid(buf, op, g, PS_DOUBLE);
+}
+void __PSReduceGridInt(void *buf, enum PSReduceOp op, __PSGrid *g) {
+  ReduceGrid(buf, op, g, PS_INT);
+}
+void __PSReduceGridLong(void *buf, enum PSReduceOp op, __PSGrid *g) {
+  ReduceGrid(buf, op, g, PS_LONG);
+}
+
+void *__PSHipStream(void) {
+  EnsureInit();
+  return (void *)rt().stream;
+}
+
+int __PSHipNumCUs(void) {
+  EnsureInit();
+  return rt().num_cus;
+}
+
+void *__PSHipScratch(size_t bytes) { return Scratch(bytes); }
+
+float __PSHipLastRunMillis(void) { return rt().last_run_ms; }
+
+}  // extern "C"

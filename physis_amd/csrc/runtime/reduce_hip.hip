@@ -1,0 +1,147 @@
+// reduce_hip: PSReduce over a whole grid on the device.
+//
+// Replaces the CUDA target's Thrust calls (reference runtime/reduce_cuda.h:
+// 19-40, reduce_cuda.cu:14-40).  Two deterministic stages:
+//   1. a fixed number of 256-thread blocks stream the grid with 16-byte
+//      loads, four in flight per thread, fold them in the element type, and
+//      reduce wave64 -> block, writing one partial per block;
+//   2. one block folds the partials in a fixed order.
+// The result is copied into pinned host memory and returned synchronously,
+// as the reference does.  Traffic: one read of num_elms * elm_size bytes.
+
+#include <cstring>
+
+#include "physis/hip/device.h"
+#include "runtime_hip.h"
+
+namespace physis_hip {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kUnroll = 4;
+
+// 16-byte native vectors (clang ext_vector_type) so that the loads lower to
+// global_load_dwordx4 and accept the nontemporal hint.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef long i64x2 __attribute__((ext_vector_type(2)));
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { typedef f32x4 V; static constexpr int N = 4; };
+template <> struct Vec16<int> { typedef i32x4 V; static constexpr int N = 4; };
+template <> struct Vec16<double> { typedef f64x2 V; static constexpr int N = 2; };
+template <> struct Vec16<long> { typedef i64x2 V; static constexpr int N = 2; };
+
+template <typename T, int OP>
+__device__ __forceinline__ T FoldVec(T acc, const typename Vec16<T>::V &v) {
+  typedef ReduceOp<T, OP> Op;
+#pragma unroll
+  for (int i = 0; i < Vec16<T>::N; ++i) acc = Op::apply(acc, v[i]);
+  return acc;
+}
+
+template <typename T, int OP>
+__global__ __launch_bounds__(kThreads) void ReduceStage1(
+    const T *__restrict__ p, int64_t n, T *__restrict__ partials) {
+  typedef ReduceOp<T, OP> Op;
+  typedef typename Vec16<T>::V V;
+  constexpr int N = Vec16<T>::N;
+  __shared__ T lds[kThreads / kWave];
+
+  const int64_t nvec = n / N;
+  const V *pv = reinterpret_cast<const V *>(p);
+  const int64_t stride = (int64_t)gridDim.x * kThreads;
+  int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  T acc = Op::identity();
+  // Four independent 16-byte loads per thread per trip keep enough bytes in
+  // flight to stream at HBM rate.
+  for (; i + (kUnroll - 1) * stride < nvec; i += kUnroll * stride) {
+    V v[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+      v[u] = __builtin_nontemporal_load(pv + i + u * stride);
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) acc = FoldVec<T, OP>(acc, v[u]);
+  }
+  for (; i < nvec; i += stride) acc = FoldVec<T, OP>(acc, pv[i]);
+  // Scalar tail, taken by the first threads of block 0.
+  if (blockIdx.x == 0) {
+    int64_t t = nvec * N + threadIdx.x;
+    if (t < n) acc = Op::apply(acc, p[t]);
+  }
+  T r = BlockReduce<T, OP, kThreads>(acc, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+template <typename T, int OP>
+__global__ __launch_bounds__(1024) void ReduceStage2(
+    const T *__restrict__ partials, int nparts, T *__restrict__ out) {
+  typedef ReduceOp<T, OP> Op;
+  __shared__ T lds[1024 / kWave];
+  T acc = Op::identity();
+  for (int i = threadIdx.x; i < nparts; i += 1024)
+    acc = Op::apply(acc, partials[i]);
+  T r = BlockReduce<T, OP, 1024>(acc, lds);
+  if (threadIdx.x == 0) out[0] = r;
+}
+
+template <typename T, int OP>
+void Launch(const T *p, int64_t n, void *host_out) {
+  Runtime &r = rt();
+  int64_t per_block = (int64_t)kThreads * Vec16<T>::N * kUnroll;
+  int64_t want = (n + per_block - 1) / per_block;
+  int64_t cap = (int64_t)r.num_cus * 8;
+  int nb = (int)(want < 1 ? 1 : (want > cap ? cap : want));
+  T *parts = (T *)Scratch(sizeof(T) * (size_t)(nb + 1));
+  T *out = parts + nb;
+  hipLaunchKernelGGL((ReduceStage1<T, OP>), dim3(nb), dim3(kThreads), 0,
+                     r.stream, p, n, parts);
+  PS_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL((ReduceStage2<T, OP>), dim3(1), dim3(1024), 0, r.stream,
+                     parts, nb, out);
+  PS_HIP_CHECK(hipGetLastError());
+  PS_HIP_CHECK(hipMemcpyAsync(host_out, out, sizeof(T), hipMemcpyDeviceToHost,
+                              r.stream));
+  PS_HIP_CHECK(hipStreamSynchronize(r.stream));
+}
+
+template <typename T>
+void Dispatch(void *buf, PSReduceOp op, const __PSGrid *g) {
+  if (g->num_elms <= 0) PS_FATAL("PSReduce on an empty grid");
+  void *slot = HostResultSlot();
+  const T *p = (const T *)g->p;
+  switch (op) {
+    case PS_MAX: Launch<T, PS_MAX>(p, g->num_elms, slot); break;
+    case PS_MIN: Launch<T, PS_MIN>(p, g->num_elms, slot); break;
+    case PS_SUM: Launch<T, PS_SUM>(p, g->num_elms, slot); break;
+    case PS_PROD: Launch<T, PS_PROD>(p, g->num_elms, slot); break;
+    default: PS_FATAL("unknown reduction operator %d", (int)op);
+  }
+  std::memcpy(buf, slot, sizeof(T));
+}
+
+}  // namespace
+
+void ReduceGrid(void *buf, PSReduceOp op, const __PSGrid *g, PSType type) {
+  EnsureInit();
+  int want_size = 0;
+  switch (type) {
+    case PS_FLOAT: want_size = sizeof(float); break;
+    case PS_DOUBLE: want_size = sizeof(double); break;
+    case PS_INT: want_size = sizeof(int); break;
+    case PS_LONG: want_size = sizeof(long); break;
+    default: PS_FATAL("PSReduce on unsupported element type %d", (int)type);
+  }
+  if (g->elm_size != want_size)
+    PS_FATAL("PSReduce: grid element size %d does not match the %d-byte "
+             "reduction type", g->elm_size, want_size);
+  switch (type) {
+    case PS_FLOAT: Dispatch<float>(buf, op, g); break;
+    case PS_DOUBLE: Dispatch<double>(buf, op, g); break;
+    case PS_INT: Dispatch<int>(buf, op, g); break;
+    case PS_LONG: Dispatch<long>(buf, op, g); break;
+    default: break;
+  }
+}
+
+}  // namespace physis_hip

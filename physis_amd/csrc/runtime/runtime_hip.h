@@ -1,0 +1,58 @@
+// Internal declarations of libphysis_rt_hip: error checking, the runtime
+// singleton and helpers shared by grid, buffer and reduction code.
+#ifndef PHYSIS_AMD_RUNTIME_HIP_H_
+#define PHYSIS_AMD_RUNTIME_HIP_H_
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdio>
+#include <cstdlib>
+
+#include "physis/physis_hip.h"
+
+// Abort-on-error contract of the reference (runtime_common_cuda.h:16-27).
+#define PS_HIP_CHECK(call)                                                   \
+  do {                                                                       \
+    hipError_t ps_err_ = (call);                                             \
+    if (ps_err_ != hipSuccess) {                                             \
+      std::fprintf(stderr, "[physis-hip] %s:%d: %s failed: %s\n", __FILE__,  \
+                   __LINE__, #call, hipGetErrorString(ps_err_));             \
+      std::exit(EXIT_FAILURE);                                               \
+    }                                                                        \
+  } while (0)
+
+#define PS_FATAL(...)                                                        \
+  do {                                                                       \
+    std::fprintf(stderr, "[physis-hip] %s:%d: ", __FILE__, __LINE__);        \
+    std::fprintf(stderr, __VA_ARGS__);                                       \
+    std::fprintf(stderr, "\n");                                              \
+    std::exit(EXIT_FAILURE);                                                 \
+  } while (0)
+
+namespace physis_hip {
+
+struct Runtime {
+  bool initialized = false;
+  int device = 0;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;  // null stream: the reference's default
+  void *scratch = nullptr;       // reduction partials, grown on demand
+  size_t scratch_bytes = 0;
+  void *host_result = nullptr;   // pinned landing slot for scalar results
+  float last_run_ms = 0.0f;
+};
+
+Runtime &rt();
+// Initialises the device lazily so that grids created before PSInit (as some
+// test programs do) still work.
+void EnsureInit();
+void *Scratch(size_t bytes);
+void *HostResultSlot();
+
+// reduce_hip.hip
+void ReduceGrid(void *buf, PSReduceOp op, const __PSGrid *g, PSType type);
+
+}  // namespace physis_hip
+
+#endif  // PHYSIS_AMD_RUNTIME_HIP_H_
