@@ -1,0 +1,285 @@
+#!/usr/bin/env python3
+"""Benchmark of the Physis HIP target on one MI355X.
+
+Step = one pass of the headline workload over synthetic input already
+resident in HBM: the 7-point diffusion at 512^3 (fp32, cc = 0.4, others
+0.1, seeded U[0,1) input, clamped boundaries), 200 sweeps run as
+PSStencilRun(m12, m21, 50) four times with PSReduce SUM and MAX after every
+50 sweeps.  value = grid points updated per second (Gpts/s) over the whole
+step.  The same line also reports the periodic variant of that run, the
+19-point Himeno Jacobi (512x256x256, 400 sweeps with a per-sweep residual
+reduction), the 7-point kernel's HBM roofline, and the REFERENCE target's
+CPU loop timed on this host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E peak
+BYTES_PER_PT_7 = 8.0           # read u, write u' (diffusion3d.h:97-100)
+BYTES_PER_PT_19 = 56.0         # 12 coefficient streams + p + p'
+N7 = 512
+SHAPE19 = (512, 256, 256)
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+class Hip:
+    """HIP events on the runtime's stream (libamdhip64 via ctypes)."""
+
+    def __init__(self, stream: int):
+        self.lib = C.CDLL("libamdhip64.so")
+        self.stream = C.c_void_p(stream)
+        for n in ("hipEventCreate", "hipEventRecord", "hipEventSynchronize",
+                  "hipEventElapsedTime", "hipDeviceSynchronize"):
+            getattr(self.lib, n).restype = C.c_int
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.lib.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e):
+        assert self.lib.hipEventRecord(e, self.stream) == 0
+
+    def ms(self, a, b) -> float:
+        assert self.lib.hipEventSynchronize(b) == 0
+        t = C.c_float()
+        assert self.lib.hipEventElapsedTime(C.byref(t), a, b) == 0
+        return t.value
+
+    def sync(self):
+        assert self.lib.hipDeviceSynchronize() == 0
+
+
+def oracle_lib():
+    so = os.path.join(ROOT, "oracle", "liboracle.so")
+    if not os.path.exists(so):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")],
+                       check=True)
+    L = C.CDLL(so)
+    FP = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    F = C.c_float
+    L.oracle_diffusion7_run.argtypes = [FP, FP] + [C.c_int] * 5 + [F] * 7
+    L.oracle_diffusion7_run_omp.argtypes = ([FP, FP] + [C.c_int] * 5 +
+                                            [F] * 7 + [C.c_int])
+    L.oracle_diffusion7_run_omp.restype = C.c_int
+    return L
+
+
+def cpu_baselines(a: np.ndarray, coef) -> tuple[dict, dict]:
+    """The REFERENCE target's sequential loop (1 core) and the MPI-OpenMP
+    target's loop shape (all host cores), both as the oracle's C
+    restatement, on a bounded prefix of the headline run."""
+    L = oracle_lib()
+    n = a.size
+    sweeps_ref, sweeps_omp = 64, 320
+    g1, g2 = a.copy(), np.empty_like(a)
+    t0 = time.perf_counter()
+    L.oracle_diffusion7_run(g1, g2, N7, N7, N7, sweeps_ref, 0, *coef)
+    t_ref = time.perf_counter() - t0
+    cores = len(os.sched_getaffinity(0))
+    div = min(cores, 16)
+    g1, g2 = a.copy(), np.empty_like(a)
+    t0 = time.perf_counter()
+    threads = L.oracle_diffusion7_run_omp(g1, g2, N7, N7, N7, sweeps_omp, 0,
+                                          *coef, div)
+    t_omp = time.perf_counter() - t0
+    ref = {"value": round(n * sweeps_ref / t_ref / 1e9, 4), "unit": "Gpts/s",
+           "cores": 1, "kind": "port",
+           "sample": f"512^3 clamped 7-pt, {sweeps_ref} sweeps "
+                     f"({t_ref:.1f} s), REFERENCE-target loop order, gcc -O3"}
+    omp = {"value": round(n * sweeps_omp / t_omp / 1e9, 4), "unit": "Gpts/s",
+           "cores": threads, "kind": "port",
+           "sample": f"512^3 clamped 7-pt, {sweeps_omp} sweeps "
+                     f"({t_omp:.1f} s), MPI-OpenMP-target z division units "
+                     f"over {threads} threads of {cores} host cores, 1 rank"}
+    return ref, omp
+
+
+def load_traffic():
+    """HBM bytes per launch of the 7-pt run kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "diffusion7_512_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} "
+              "is not supported (the multi-GPU z-slab design is in DESIGN.md)",
+              file=sys.stderr)
+        return 2
+
+    from physis_amd import runtime as ps
+    from physis_amd.configs import (DIFFUSION_COEF, diffusion_input,
+                                    himeno_inputs)
+    from physis_amd.programs import Diffusion7, Himeno
+
+    ps.PSInit()
+    hip = Hip(ps.hip_stream())
+    d7 = Diffusion7()
+    shape = (N7, N7, N7)
+    npts = N7 ** 3
+    log("generating 512^3 input")
+    a = diffusion_input(*shape, seed=0)
+    g1 = ps.PSGridNew(shape)
+    g2 = ps.PSGridNew(shape)
+
+    def step(periodic=False, kernel_ms=None):
+        for _ in range(4):
+            if kernel_ms is not None:
+                e0, e1 = hip.event(), hip.event()
+                hip.record(e0)
+            d7.run(g1, g2, 50, DIFFUSION_COEF, periodic=periodic)
+            if kernel_ms is not None:
+                hip.record(e1)
+                kernel_ms.append(hip.ms(e0, e1) / 50.0)
+            ps.PSReduce(ps.PS_SUM, g1)
+            ps.PSReduce(ps.PS_MAX, g1)
+
+    ps.PSGridCopyin(g1, a)
+    ps.PSGridCopyin(g2, a)
+    for _ in range(args.warmup):
+        step()
+    log(f"timing {args.steps} steps")
+    hip.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    hip.sync()
+    total = time.perf_counter() - t0
+    ms_per_step = total / args.steps * 1e3
+    value = npts * 200 * args.steps / total / 1e9
+
+    # Kernel-only duration (HIP events around each 50-sweep run).
+    kernel_ms = []
+    step(kernel_ms=kernel_ms)
+    k_ms = float(np.mean(kernel_ms))
+    achieved = BYTES_PER_PT_7 * npts / (k_ms * 1e-3) / 1e9
+
+    # Periodic variant of the same run.
+    ps.PSGridCopyin(g1, a)
+    ps.PSGridCopyin(g2, a)
+    step(periodic=True)
+    hip.sync()
+    t0 = time.perf_counter()
+    step(periodic=True)
+    hip.sync()
+    t_per = time.perf_counter() - t0
+    pk = []
+    step(periodic=True, kernel_ms=pk)
+    ps.PSGridFree(g1)
+    ps.PSGridFree(g2)
+
+    # 19-point Himeno, 512x256x256, 400 sweeps with per-sweep gosa.
+    log("generating Himeno 512x256x256 inputs")
+    coefs, p = himeno_inputs(*SHAPE19, seed=0)
+    him = Himeno()
+    cg = []
+    for c in coefs:
+        g = ps.PSGridNew(SHAPE19)
+        ps.PSGridCopyin(g, c)
+        cg.append(g)
+    p0 = ps.PSGridNew(SHAPE19)
+    p1 = ps.PSGridNew(SHAPE19)
+    ps.PSGridCopyin(p0, p)
+    ps.PSGridCopyin(p1, p)
+    him.run_gosa(20, p0, p1, cg)
+    hip.sync()
+    t0 = time.perf_counter()
+    gosa = him.run_gosa(400, p0, p1, cg)
+    hip.sync()
+    t19 = time.perf_counter() - t0
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0)
+    him.run(100, p0, p1, cg)
+    hip.record(e1)
+    k19_ms = hip.ms(e0, e1) / 100.0
+    n19 = int(np.prod(SHAPE19))
+    for g in cg + [p0, p1]:
+        ps.PSGridFree(g)
+
+    line = {
+        "metric": "Gpts/s + % HBM peak (512³ 7-pt/19-pt fp32); "
+                  "time-to-residual 1e-6 vs plain",
+        "value": round(value, 2),
+        "unit": "Gpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (splitmix64-seeded U[0,1), seed 0)",
+        "config": {
+            "workload": "7-point diffusion 512x512x512 fp32, clamped "
+                        "boundaries, cc=0.4 others 0.1, 200 sweeps as "
+                        "4 x PSStencilRun(m12, m21, 25) ... with PSReduce "
+                        "SUM and MAX after every 50 sweeps",
+            "grid": [N7, N7, N7], "sweeps_per_step": 200,
+            "parallelism": "single GPU"},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "__PSStencilRun_diffusion7_kernel",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": load_traffic(),
+            "kernel_ms": round(k_ms, 5),
+            "algorithmic_bytes_per_launch": int(BYTES_PER_PT_7 * npts),
+        },
+        "periodic_7pt": {
+            "value": round(npts * 200 / t_per / 1e9, 2), "unit": "Gpts/s",
+            "kernel_ms": round(float(np.mean(pk)), 5),
+            "hbm_frac": round(BYTES_PER_PT_7 * npts /
+                              (float(np.mean(pk)) * 1e-3) / 1e9 /
+                              HBM_PEAK_GBS, 4)},
+        "himeno_19pt": {
+            "grid": list(SHAPE19), "sweeps": 400,
+            "value": round(n19 * 400 / t19 / 1e9, 2), "unit": "Gpts/s",
+            "kernel_ms": round(k19_ms, 5),
+            "hbm_frac": round(BYTES_PER_PT_19 * n19 / (k19_ms * 1e-3) / 1e9 /
+                              HBM_PEAK_GBS, 4),
+            "gosa_first_last": [float(gosa[0]), float(gosa[-1])]},
+    }
+    if not args.no_cpu:
+        log("timing the CPU baselines")
+        ref, omp = cpu_baselines(a, DIFFUSION_COEF)
+        line["cpu_baseline"] = ref
+        line["cpu_baseline_mpi_openmp"] = omp
+    print(json.dumps(line), flush=True)
+    ps.PSFinalize()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

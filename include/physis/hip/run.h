@@ -143,6 +143,47 @@ class RunTimer {
   hipEvent_t start_ = nullptr, stop_ = nullptr;
 };
 
+// Partial results of a PSStencilMapReduce map inside one run function.  The
+// run kernel of execution i writes one partial per block into Slot(i);
+// After(i) folds every completed batch of executions into the user's array
+// (out[i] = reduction of execution i).  A NULL out makes it a no-op.
+class MapReducer {
+ public:
+  MapReducer(void *out, int op, int type, int nblocks, int iter, int map_index,
+             int num_maps)
+      : out_((char *)out), op_(op), type_(type), nblocks_(nblocks),
+        iter_(iter) {
+    if (!out_ || iter <= 0 || nblocks <= 0) {
+      out_ = nullptr;
+      return;
+    }
+    elm_ = (type == PS_DOUBLE || type == PS_LONG) ? 8 : 4;
+    batch_ = iter < kBatch ? iter : kBatch;
+    (void)num_maps;
+    base_ = (char *)__PSHipScratch(1 + map_index % 7,
+                                   (size_t)batch_ * nblocks_ * elm_);
+  }
+  void *Slot(int i) const {
+    if (!out_) return nullptr;
+    return base_ + (size_t)(i % batch_) * nblocks_ * elm_;
+  }
+  void After(int i) {
+    if (!out_) return;
+    if ((i + 1) % batch_ != 0 && i + 1 != iter_) return;
+    const int first = i - (i % batch_);
+    const int count = i + 1 - first;
+    __PSHipReduceFold(type_, op_, base_, nblocks_, count,
+                      out_ + (size_t)first * elm_);
+  }
+
+ private:
+  static constexpr int kBatch = 256;
+  char *out_;
+  int op_, type_, nblocks_, iter_;
+  int elm_ = 4, batch_ = 1;
+  char *base_ = nullptr;
+};
+
 }  // namespace physis_hip
 
 #endif  // PHYSIS_HIP_RUN_H_

@@ -218,13 +218,27 @@ struct Stage {
   }
 
   // Issues the two loads of plane z for the wave's row y = y0 - HY + wave.
-  __device__ __forceinline__ void Load(const BufWin<T> &w, const int *dim,
-                                       int x0, int y, int z, int lane) {
-    w.template Load<VX>(Offset(dim, x0 + lane * VX, y, z), own);
+  // Non-periodic reads go through the block's window w (the REF linear rule
+  // may spill into the neighbouring plane); periodic reads go through a
+  // window of exactly the wrapped plane, which may lie anywhere in the grid.
+  __device__ __forceinline__ void Load(const BufWin<T> &w, const T *p,
+                                       const int *dim, int x0, int y, int z,
+                                       int lane) {
     const int hl = lane < 2 * HX ? lane : 0;
     const int hx = hl < HX ? x0 - HX + hl : x0 + kTX + (hl - HX);
     T v[1];
-    w.template Load<1>(Offset(dim, hx, y, z), v);
+    if (PERIODIC) {
+      const int nxy = dim[0] * dim[1];
+      const int zw = WrapIndex(z, dim[2]);
+      const BufWin<T> pw =
+          BufWin<T>::Make(p, nxy * dim[2], (long)zw * nxy, (long)(zw + 1) * nxy);
+      const int yw = WrapIndex(y, dim[1]) * dim[0] + zw * nxy;
+      pw.template Load<VX>(WrapIndex(x0 + lane * VX, dim[0]) + yw, own);
+      pw.template Load<1>(WrapIndex(hx, dim[0]) + yw, v);
+    } else {
+      w.template Load<VX>(Offset(dim, x0 + lane * VX, y, z), own);
+      w.template Load<1>(Offset(dim, hx, y, z), v);
+    }
     halo = v[0];
   }
 

@@ -79,6 +79,17 @@ extern PSStencil __PSStencilMapRed(void *, ...);
 extern PSStencil __PSStencilMapBlack(void *, ...);
 extern void PSStencilRun(PSStencil, ...);
 
+/* HIP target extension: a stencil map whose kernel returns a value per
+ * point.  Each execution i of the map inside a PSStencilRun reduces those
+ * values with op and stores the result in v[i] (v holds at least iter
+ * elements of the kernel's return type).  It is the reduction-kernel form
+ * documented, but never implemented, for PSReduce in the reference
+ * (docs/programming.md:283-315), fused into the sweep. */
+#define PSStencilMapReduce(v, op, k, ...) \
+  __PSStencilMapReduce((void *)(v), op, (void *)k, __VA_ARGS__)
+extern PSStencil __PSStencilMapReduce(void *v, enum PSReduceOp op, void *k,
+                                      ...);
+
 /* PSReduce(&v, op, grid), or the kernel form
  * PSReduce(&v, op, kernel, domain, args...) documented in the Physis
  * programming guide (docs/programming.md:283-315). */

@@ -39,7 +39,7 @@ RUNTIME_SOURCES = [
 ]
 
 # Translated programs built into their own shared libraries.
-PROGRAM_NAMES = ["diffusion7"]
+PROGRAM_NAMES = ["diffusion7", "himeno"]
 
 
 def _run(cmd: list[str]) -> None:

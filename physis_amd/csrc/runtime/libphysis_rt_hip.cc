@@ -38,19 +38,21 @@ void EnsureInit() {
   r.initialized = true;
 }
 
-void *Scratch(size_t bytes) {
+void *Scratch(int slot, size_t bytes) {
   Runtime &r = rt();
   EnsureInit();
-  if (bytes > r.scratch_bytes) {
-    if (r.scratch) {
+  if (slot < 0 || slot >= Runtime::kScratchSlots)
+    PS_FATAL("scratch slot %d out of range", slot);
+  if (bytes > r.scratch_bytes[slot]) {
+    if (r.scratch[slot]) {
       PS_HIP_CHECK(hipStreamSynchronize(r.stream));
-      PS_HIP_CHECK(hipFree(r.scratch));
+      PS_HIP_CHECK(hipFree(r.scratch[slot]));
     }
     size_t cap = bytes < (1u << 20) ? (1u << 20) : bytes;
-    PS_HIP_CHECK(hipMalloc(&r.scratch, cap));
-    r.scratch_bytes = cap;
+    PS_HIP_CHECK(hipMalloc(&r.scratch[slot], cap));
+    r.scratch_bytes[slot] = cap;
   }
-  return r.scratch;
+  return r.scratch[slot];
 }
 
 void *HostResultSlot() {
@@ -117,10 +119,12 @@ void PSFinalize(void) {
   Runtime &r = rt();
   if (!r.initialized) return;
   PS_HIP_CHECK(hipDeviceSynchronize());
-  if (r.scratch) PS_HIP_CHECK(hipFree(r.scratch));
+  for (int i = 0; i < Runtime::kScratchSlots; ++i) {
+    if (r.scratch[i]) PS_HIP_CHECK(hipFree(r.scratch[i]));
+    r.scratch[i] = nullptr;
+    r.scratch_bytes[i] = 0;
+  }
   if (r.host_result) PS_HIP_CHECK(hipHostFree(r.host_result));
-  r.scratch = nullptr;
-  r.scratch_bytes = 0;
   r.host_result = nullptr;
 }
 
@@ -297,8 +301,6 @@ int __PSHipNumCUs(void) {
   return rt().num_cus;
 }
 
-void *__PSHipScratch(size_t bytes) { return Scratch(bytes); }
-
-float __PSHipLastRunMillis(void) { return rt().last_run_ms; }
+void *__PSHipScratch(int slot, size_t bytes) { return Scratch(slot, bytes); }
 
 }  // extern "C"

@@ -3,9 +3,13 @@
 // Replaces the CUDA target's Thrust calls (reference runtime/reduce_cuda.h:
 // 19-40, reduce_cuda.cu:14-40).  Two deterministic stages:
 //   1. a fixed number of 256-thread blocks stream the grid with 16-byte
-//      loads, four in flight per thread, fold them in the element type, and
-//      reduce wave64 -> block, writing one partial per block;
+//      loads, four in flight per thread, fold them (float SUM/PROD in
+//      double, everything else in the element type), and reduce wave64 ->
+//      block, writing one partial per block;
 //   2. one block folds the partials in a fixed order.
+// Float sums are therefore the FP64 sum rounded once to float; the REF
+// target's sequential float loop (libphysis_rt_ref.cc:19-30) drifts from
+// that value as the running sum grows.
 // The result is copied into pinned host memory and returned synchronously,
 // as the reference does.  Traffic: one read of num_elms * elm_size bytes.
 
@@ -32,27 +36,37 @@ template <> struct Vec16<int> { typedef i32x4 V; static constexpr int N = 4; };
 template <> struct Vec16<double> { typedef f64x2 V; static constexpr int N = 2; };
 template <> struct Vec16<long> { typedef i64x2 V; static constexpr int N = 2; };
 
+// Accumulation type: float SUM and PROD fold in double, so the result is the
+// FP64 value rounded once to float, independent of the fold order.  Other
+// cases fold in the element type (exact for MAX/MIN, wrapping for integers).
+template <typename T, int OP> struct Acc { typedef T A; };
+template <> struct Acc<float, PS_SUM> { typedef double A; };
+template <> struct Acc<float, PS_PROD> { typedef double A; };
+
 template <typename T, int OP>
-__device__ __forceinline__ T FoldVec(T acc, const typename Vec16<T>::V &v) {
-  typedef ReduceOp<T, OP> Op;
+__device__ __forceinline__ typename Acc<T, OP>::A FoldVec(
+    typename Acc<T, OP>::A acc, const typename Vec16<T>::V &v) {
+  typedef typename Acc<T, OP>::A A;
+  typedef ReduceOp<A, OP> Op;
 #pragma unroll
-  for (int i = 0; i < Vec16<T>::N; ++i) acc = Op::apply(acc, v[i]);
+  for (int i = 0; i < Vec16<T>::N; ++i) acc = Op::apply(acc, (A)v[i]);
   return acc;
 }
 
 template <typename T, int OP>
 __global__ __launch_bounds__(kThreads) void ReduceStage1(
-    const T *__restrict__ p, int64_t n, T *__restrict__ partials) {
-  typedef ReduceOp<T, OP> Op;
+    const T *__restrict__ p, int64_t n, typename Acc<T, OP>::A *__restrict__ partials) {
+  typedef typename Acc<T, OP>::A A;
+  typedef ReduceOp<A, OP> Op;
   typedef typename Vec16<T>::V V;
   constexpr int N = Vec16<T>::N;
-  __shared__ T lds[kThreads / kWave];
+  __shared__ A lds[kThreads / kWave];
 
   const int64_t nvec = n / N;
   const V *pv = reinterpret_cast<const V *>(p);
   const int64_t stride = (int64_t)gridDim.x * kThreads;
   int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  T acc = Op::identity();
+  A acc = Op::identity();
   // Four independent 16-byte loads per thread per trip keep enough bytes in
   // flight to stream at HBM rate.
   for (; i + (kUnroll - 1) * stride < nvec; i += kUnroll * stride) {
@@ -67,22 +81,24 @@ __global__ __launch_bounds__(kThreads) void ReduceStage1(
   // Scalar tail, taken by the first threads of block 0.
   if (blockIdx.x == 0) {
     int64_t t = nvec * N + threadIdx.x;
-    if (t < n) acc = Op::apply(acc, p[t]);
+    if (t < n) acc = Op::apply(acc, (A)p[t]);
   }
-  T r = BlockReduce<T, OP, kThreads>(acc, lds);
+  A r = BlockReduce<A, OP, kThreads>(acc, lds);
   if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
 
 template <typename T, int OP>
 __global__ __launch_bounds__(1024) void ReduceStage2(
-    const T *__restrict__ partials, int nparts, T *__restrict__ out) {
-  typedef ReduceOp<T, OP> Op;
-  __shared__ T lds[1024 / kWave];
-  T acc = Op::identity();
+    const typename Acc<T, OP>::A *__restrict__ partials, int nparts,
+    T *__restrict__ out) {
+  typedef typename Acc<T, OP>::A A;
+  typedef ReduceOp<A, OP> Op;
+  __shared__ A lds[1024 / kWave];
+  A acc = Op::identity();
   for (int i = threadIdx.x; i < nparts; i += 1024)
     acc = Op::apply(acc, partials[i]);
-  T r = BlockReduce<T, OP, 1024>(acc, lds);
-  if (threadIdx.x == 0) out[0] = r;
+  A r = BlockReduce<A, OP, 1024>(acc, lds);
+  if (threadIdx.x == 0) out[0] = (T)r;
 }
 
 template <typename T, int OP>
@@ -92,8 +108,9 @@ void Launch(const T *p, int64_t n, void *host_out) {
   int64_t want = (n + per_block - 1) / per_block;
   int64_t cap = (int64_t)r.num_cus * 8;
   int nb = (int)(want < 1 ? 1 : (want > cap ? cap : want));
-  T *parts = (T *)Scratch(sizeof(T) * (size_t)(nb + 1));
-  T *out = parts + nb;
+  typedef typename Acc<T, OP>::A A;
+  A *parts = (A *)Scratch(0, sizeof(A) * (size_t)(nb + 1));
+  T *out = (T *)(parts + nb);
   hipLaunchKernelGGL((ReduceStage1<T, OP>), dim3(nb), dim3(kThreads), 0,
                      r.stream, p, n, parts);
   PS_HIP_CHECK(hipGetLastError());
@@ -144,4 +161,68 @@ void ReduceGrid(void *buf, PSReduceOp op, const __PSGrid *g, PSType type) {
   }
 }
 
+// -------------------------------------------- partials of fused reductions
+// Run kernels of PSStencilMapReduce maps leave one partial per block for each
+// execution; this folds row r of `partials` (nparts values) into out[r], in
+// a fixed order, accumulating float data in double.
+template <typename T> struct FoldAcc { typedef T A; };
+template <> struct FoldAcc<float> { typedef double A; };
+
+template <typename T, int OP>
+__global__ __launch_bounds__(256) void FoldRows(const T *__restrict__ partials,
+                                                int nparts,
+                                                T *__restrict__ out) {
+  typedef typename FoldAcc<T>::A A;
+  typedef ReduceOp<A, OP> Op;
+  __shared__ A lds[256 / kWave];
+  const T *row = partials + (size_t)blockIdx.x * nparts;
+  A acc = Op::identity();
+  for (int i = threadIdx.x; i < nparts; i += 256)
+    acc = Op::apply(acc, (A)row[i]);
+  A r = BlockReduce<A, OP, 256>(acc, lds);
+  if (threadIdx.x == 0) out[blockIdx.x] = (T)r;
+}
+
+template <typename T>
+void FoldTyped(int op, const void *partials, int nparts, int count,
+               void *host_out) {
+  Runtime &r = rt();
+  // Results land after the partials in the scratch area of the caller, so
+  // they use a separate small device buffer.
+  static T *dev_out = nullptr;
+  static int dev_cap = 0;
+  if (count > dev_cap) {
+    if (dev_out) PS_HIP_CHECK(hipFree(dev_out));
+    PS_HIP_CHECK(hipMalloc(&dev_out, sizeof(T) * (size_t)count));
+    dev_cap = count;
+  }
+  const T *p = (const T *)partials;
+  switch (op) {
+    case PS_MAX: hipLaunchKernelGGL((FoldRows<T, PS_MAX>), dim3(count), dim3(256), 0, r.stream, p, nparts, dev_out); break;
+    case PS_MIN: hipLaunchKernelGGL((FoldRows<T, PS_MIN>), dim3(count), dim3(256), 0, r.stream, p, nparts, dev_out); break;
+    case PS_SUM: hipLaunchKernelGGL((FoldRows<T, PS_SUM>), dim3(count), dim3(256), 0, r.stream, p, nparts, dev_out); break;
+    case PS_PROD: hipLaunchKernelGGL((FoldRows<T, PS_PROD>), dim3(count), dim3(256), 0, r.stream, p, nparts, dev_out); break;
+    default: PS_FATAL("unknown reduction operator %d", op);
+  }
+  PS_HIP_CHECK(hipGetLastError());
+  PS_HIP_CHECK(hipMemcpyAsync(host_out, dev_out, sizeof(T) * (size_t)count,
+                              hipMemcpyDeviceToHost, r.stream));
+  PS_HIP_CHECK(hipStreamSynchronize(r.stream));
+}
+
 }  // namespace physis_hip
+
+extern "C" void __PSHipReduceFold(int type, int op, const void *partials,
+                                  int nparts, int count, void *host_out) {
+  using namespace physis_hip;
+  EnsureInit();
+  if (count <= 0) return;
+  if (nparts <= 0) PS_FATAL("reduction over an empty domain");
+  switch (type) {
+    case PS_FLOAT: FoldTyped<float>(op, partials, nparts, count, host_out); break;
+    case PS_DOUBLE: FoldTyped<double>(op, partials, nparts, count, host_out); break;
+    case PS_INT: FoldTyped<int>(op, partials, nparts, count, host_out); break;
+    case PS_LONG: FoldTyped<long>(op, partials, nparts, count, host_out); break;
+    default: PS_FATAL("unsupported reduction type %d", type);
+  }
+}

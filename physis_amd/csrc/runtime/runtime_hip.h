@@ -37,17 +37,17 @@ struct Runtime {
   int device = 0;
   int num_cus = 0;
   hipStream_t stream = nullptr;  // null stream: the reference's default
-  void *scratch = nullptr;       // reduction partials, grown on demand
-  size_t scratch_bytes = 0;
+  static constexpr int kScratchSlots = 8;
+  void *scratch[kScratchSlots] = {};  // partial results, grown on demand
+  size_t scratch_bytes[kScratchSlots] = {};
   void *host_result = nullptr;   // pinned landing slot for scalar results
-  float last_run_ms = 0.0f;
 };
 
 Runtime &rt();
 // Initialises the device lazily so that grids created before PSInit (as some
 // test programs do) still work.
 void EnsureInit();
-void *Scratch(size_t bytes);
+void *Scratch(int slot, size_t bytes);
 void *HostResultSlot();
 
 // reduce_hip.hip

@@ -1,0 +1,75 @@
+"""Typed ctypes bindings of the translated benchmark programs.
+
+Each program is a Physis DSL source under physis_amd/programs/ whose
+translation (<name>.hip.cpp) is built into libphysis_prog_<name>.so; the
+functions bound here are the program's own C functions (the ones a Physis
+user would call), taking grid handles from physis_amd.runtime.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import runtime as rt
+
+_F = C.c_float
+
+
+class Diffusion7:
+    """physis_amd/programs/diffusion7.c: 7-point diffusion, clamped or
+    periodic boundaries."""
+
+    def __init__(self):
+        L = rt.load_program("diffusion7")
+        G = rt.PSGridPtr
+        L.diffusion7_run.argtypes = [G, G, C.c_int] + [_F] * 7
+        L.diffusion7_periodic_run.argtypes = [G, G, C.c_int] + [_F] * 7
+        L.diffusion7_sweep_domain.argtypes = ([G, G] + [C.c_int] * 7 +
+                                              [_F] * 7)
+        self.lib = L
+
+    def run(self, g1, g2, count: int, coef, periodic: bool = False) -> None:
+        """count sweeps g1 -> g2 -> g1 over the whole grid (count even).
+        coef = (ce, cw, cn, cs, ct, cb, cc)."""
+        fn = self.lib.diffusion7_periodic_run if periodic else \
+            self.lib.diffusion7_run
+        fn(g1.handle, g2.handle, count, *[float(c) for c in coef])
+
+    def sweep_domain(self, g1, g2, dom, coef, periodic: bool = False) -> None:
+        """One sweep g1 -> g2 over dom = (x0, x1, y0, y1, z0, z1)."""
+        self.lib.diffusion7_sweep_domain(g1.handle, g2.handle, *dom,
+                                         int(periodic),
+                                         *[float(c) for c in coef])
+
+
+class Himeno:
+    """physis_amd/programs/himeno.c: 19-point Himeno Jacobi sweeps."""
+
+    COEFS = ("a0", "a1", "a2", "a3", "b0", "b1", "b2", "c0", "c1", "c2",
+             "bnd", "wrk1")
+
+    def __init__(self):
+        L = rt.load_program("himeno")
+        G = rt.PSGridPtr
+        L.himeno_run.argtypes = [C.c_int] + [G] * 14 + [_F]
+        L.himeno_run_gosa.argtypes = ([C.c_int] + [G] * 14 + [_F] +
+                                      [C.c_void_p, C.c_void_p])
+        self.lib = L
+
+    def run(self, nn: int, p0, p1, coefs, omega: float = 0.8) -> None:
+        self.lib.himeno_run(nn, p0.handle, p1.handle,
+                            *[c.handle for c in coefs], float(omega))
+
+    def run_gosa(self, nn: int, p0, p1, coefs,
+                 omega: float = 0.8) -> np.ndarray:
+        """nn sweeps; returns the float residual sum of each sweep."""
+        even = np.zeros(nn // 2, np.float32)
+        odd = np.zeros(nn // 2, np.float32)
+        self.lib.himeno_run_gosa(nn, p0.handle, p1.handle,
+                                 *[c.handle for c in coefs], float(omega),
+                                 even.ctypes.data, odd.ctypes.data)
+        out = np.empty(nn // 2 * 2, np.float32)
+        out[0::2] = even
+        out[1::2] = odd
+        return out

@@ -28,6 +28,25 @@ static void diffusion7_kernel(const int x, const int y, const int z,
   PSGridEmit(g2, cc*c + cw*w + ce*e + cs*s + cn*n + cb*b + ct*t);
 }
 
+/* The same diffusion with periodic wrap on every axis: every neighbour is
+ * read with PSGridGetPeriodic (wrap semantics of
+ * tests/system_tests/test_cases/test_7-pt-periodic.c:12-24), operand order
+ * as above. */
+static void diffusion7_periodic_kernel(const int x, const int y, const int z,
+                                       PSGrid3DFloat g1, PSGrid3DFloat g2,
+                                       float ce, float cw, float cn, float cs,
+                                       float ct, float cb, float cc) {
+  float c, w, e, n, s, b, t;
+  c = PSGridGet(g1, x, y, z);
+  w = PSGridGetPeriodic(g1, x-1, y, z);
+  e = PSGridGetPeriodic(g1, x+1, y, z);
+  n = PSGridGetPeriodic(g1, x, y-1, z);
+  s = PSGridGetPeriodic(g1, x, y+1, z);
+  b = PSGridGetPeriodic(g1, x, y, z-1);
+  t = PSGridGetPeriodic(g1, x, y, z+1);
+  PSGridEmit(g2, cc*c + cw*w + ce*e + cs*s + cn*n + cb*b + ct*t);
+}
+
 /* count sweeps over the whole grid, ping-ponging g1 -> g2 -> g1; count is
  * even, so the result ends in g1. */
 void diffusion7_run(PSGrid3DFloat g1, PSGrid3DFloat g2, int count,
@@ -40,4 +59,31 @@ void diffusion7_run(PSGrid3DFloat g1, PSGrid3DFloat g2, int count,
                PSStencilMap(diffusion7_kernel, dom, g2, g1,
                             ce, cw, cn, cs, ct, cb, cc),
                count / 2);
+}
+
+void diffusion7_periodic_run(PSGrid3DFloat g1, PSGrid3DFloat g2, int count,
+                             float ce, float cw, float cn, float cs,
+                             float ct, float cb, float cc) {
+  PSDomain3D dom = PSDomain3DNew(0, PSGridDim(g1, 0), 0, PSGridDim(g1, 1),
+                                 0, PSGridDim(g1, 2));
+  PSStencilRun(PSStencilMap(diffusion7_periodic_kernel, dom, g1, g2,
+                            ce, cw, cn, cs, ct, cb, cc),
+               PSStencilMap(diffusion7_periodic_kernel, dom, g2, g1,
+                            ce, cw, cn, cs, ct, cb, cc),
+               count / 2);
+}
+
+/* One sweep g1 -> g2 over an arbitrary sub-domain (used to check domain
+ * handling: points outside the domain keep g2's previous contents). */
+void diffusion7_sweep_domain(PSGrid3DFloat g1, PSGrid3DFloat g2,
+                             int x0, int x1, int y0, int y1, int z0, int z1,
+                             int periodic, float ce, float cw, float cn,
+                             float cs, float ct, float cb, float cc) {
+  PSDomain3D dom = PSDomain3DNew(x0, x1, y0, y1, z0, z1);
+  if (periodic)
+    PSStencilRun(PSStencilMap(diffusion7_periodic_kernel, dom, g1, g2,
+                              ce, cw, cn, cs, ct, cb, cc));
+  else
+    PSStencilRun(PSStencilMap(diffusion7_kernel, dom, g1, g2,
+                              ce, cw, cn, cs, ct, cb, cc));
 }
