@@ -107,9 +107,12 @@ __device__ __forceinline__ T BlockReduce(T v, T *lds) {
 // Hardware dispatch places block b on XCD (b % 8).  Renumbering so that each
 // XCD receives a contiguous range of logical tiles keeps a tile's y- and
 // z-neighbours (whose planes it re-reads as halo) in the same L2.
+// With nblocks = 8q + r, XCD j receives q + (j < r) blocks; its range of
+// logical ids starts at j*q + min(j, r).  A bijection for every nblocks.
 __device__ __forceinline__ int XcdLogicalBlock(int b, int nblocks) {
-  if (nblocks % kNumXCD != 0) return b;
-  return (b % kNumXCD) * (nblocks / kNumXCD) + b / kNumXCD;
+  const int q = nblocks / kNumXCD, r = nblocks % kNumXCD;
+  const int xcd = b % kNumXCD;
+  return xcd * q + (xcd < r ? xcd : r) + b / kNumXCD;
 }
 
 // ----------------------------------------------------------- grid sampling

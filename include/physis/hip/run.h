@@ -15,6 +15,23 @@
 
 namespace physis_hip {
 
+// ------------------------------------------------ host intrinsic helpers
+// PSGrid<N>D<T>New(d0, d1, d2): the type-info block the reference emits
+// inline (reference_runtime_builder.cc:959-1065) folded into one call.
+inline __PSGrid *NewGrid(int type, int size, int num_dims, int d0, int d1 = 1,
+                         int d2 = 1) {
+  __PSGridTypeInfo ti = {type, size, 0, nullptr};
+  PSVectorInt dim = {d0, d1, d2};
+  return __PSGridNew(&ti, num_dims, dim, nullptr);
+}
+
+// PSGridSet(g, i..., v) with v converted to the element type.
+template <typename T>
+inline void SetElem(__PSGrid *g, T v, PSIndex i0, PSIndex i1 = 0,
+                    PSIndex i2 = 0) {
+  __PSGridSet(g, &v, i0, i1, i2);
+}
+
 // Resident blocks of `kernel` per compute unit at `threads` per block.
 inline int BlocksPerCU(const void *kernel, int threads) {
   int n = 0;

@@ -98,7 +98,13 @@ struct BufWin {
   __device__ __forceinline__ void Load(int off, T (&out)[N]) const {
     constexpr int kBytes = (int)sizeof(T) * N;
     const int b = Bytes(off);
-    if constexpr (kBytes == 16) {
+    if constexpr (kBytes == 32) {
+      auto v0 = __builtin_amdgcn_raw_buffer_load_b128(rsrc, b, 0, 0);
+      auto v1 = __builtin_amdgcn_raw_buffer_load_b128(
+          rsrc, b == kDrop ? kDrop : b + 16, 0, 0);
+      __builtin_memcpy(out, &v0, 16);
+      __builtin_memcpy((char *)out + 16, &v1, 16);
+    } else if constexpr (kBytes == 16) {
       auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, b, 0, 0);
       __builtin_memcpy(out, &v, 16);
     } else if constexpr (kBytes == 8) {
@@ -123,7 +129,14 @@ struct BufWin {
   __device__ __forceinline__ void Store(int off, const T (&v)[N]) const {
     constexpr int kBytes = (int)sizeof(T) * N;
     const int b = Bytes(off);
-    if constexpr (kBytes == 16) {
+    if constexpr (kBytes == 32) {
+      __attribute__((ext_vector_type(4))) unsigned w0, w1;
+      __builtin_memcpy(&w0, v, 16);
+      __builtin_memcpy(&w1, (const char *)v + 16, 16);
+      __builtin_amdgcn_raw_buffer_store_b128(w0, rsrc, b, 0, AUX);
+      __builtin_amdgcn_raw_buffer_store_b128(
+          w1, rsrc, b == kDrop ? kDrop : b + 16, 0, AUX);
+    } else if constexpr (kBytes == 16) {
       __attribute__((ext_vector_type(4))) unsigned w;
       __builtin_memcpy(&w, v, 16);
       __builtin_amdgcn_raw_buffer_store_b128(w, rsrc, b, 0, AUX);
@@ -289,6 +302,28 @@ struct Stage {
     }
   }
 };
+
+// ----------------------------------------------------- irregular accesses
+// PSGridGet / PSGridGetPeriodic at indices the stencil analysis could not
+// reduce to constant offsets: a direct global read with the same rules as
+// the staged path (linear offset inside the grid, else 0; periodic wraps).
+template <typename T>
+__device__ __forceinline__ T GetLinear(const T *p, const int *dim, int nd,
+                                       int i0, int i1 = 0, int i2 = 0) {
+  long off = i0;
+  long n = dim[0];
+  if (nd > 1) { off += (long)i1 * dim[0]; n *= dim[1]; }
+  if (nd > 2) { off += (long)i2 * dim[0] * dim[1]; n *= dim[2]; }
+  return (off >= 0 && off < n) ? p[off] : T(0);
+}
+template <typename T>
+__device__ __forceinline__ T GetPeriodic(const T *p, const int *dim, int nd,
+                                         int i0, int i1 = 0, int i2 = 0) {
+  long off = WrapIndex(i0, dim[0]);
+  if (nd > 1) off += (long)WrapIndex(i1, dim[1]) * dim[0];
+  if (nd > 2) off += (long)WrapIndex(i2, dim[2]) * dim[0] * dim[1];
+  return p[off];
+}
 
 // Slot of the plane `k` planes past the oldest one in a ring whose oldest
 // plane sits in slot `base`.

@@ -38,8 +38,16 @@ RUNTIME_SOURCES = [
     os.path.join(CSRC, "runtime", "buffer_hip.hip"),
 ]
 
-# Translated programs built into their own shared libraries.
-PROGRAM_NAMES = ["diffusion7", "himeno"]
+TRANSLATOR_SOURCES = [os.path.join(CSRC, "translator", f) for f in
+                      ("lexer.cc", "parser.cc", "translator.cc", "physisc.cc")]
+PHYSISC = os.path.join(PKG, "bin", "physisc-hip")
+
+# Physis programs translated by physisc-hip and built into their own shared
+# libraries: the benchmark programs, and the test programs (own versions of
+# the reference's system tests) under tests/programs.
+PROGRAM_SOURCES = [os.path.join(PROGRAMS, "diffusion7.c"),
+                   os.path.join(PROGRAMS, "himeno.c"),
+                   os.path.join(ROOT, "tests", "programs", "systests.c")]
 
 
 def _run(cmd: list[str]) -> None:
@@ -73,11 +81,26 @@ def build_runtime(objdir: str) -> str:
     return out
 
 
-def build_program(name: str) -> str:
-    src = os.path.join(PROGRAMS, f"{name}.hip.cpp")
+def build_translator() -> str:
+    os.makedirs(os.path.dirname(PHYSISC), exist_ok=True)
+    _run(["g++", "-std=c++17", "-O2", "-Wall", *TRANSLATOR_SOURCES, "-o",
+          PHYSISC])
+    return PHYSISC
+
+
+def translate(src: str) -> str:
+    """physisc-hip src.c -> src.hip.cpp next to it."""
+    out = src[:-2] + ".hip.cpp"
+    _run([PHYSISC, src, "-o", out])
+    return out
+
+
+def build_program(src: str) -> str:
+    name = os.path.basename(src)[:-2]
+    gen = translate(src)
     out = os.path.join(LIB, f"libphysis_prog_{name}.so")
-    _run([HIPCC, *HIP_FLAGS, "-shared", src, "-o", out, f"-L{LIB}",
-          "-lphysis_rt_hip", "-Wl,-rpath,$ORIGIN"])
+    _run([HIPCC, *HIP_FLAGS, "-Wno-unused-variable", "-shared", gen, "-o", out,
+          f"-L{LIB}", "-lphysis_rt_hip", "-Wl,-rpath,$ORIGIN"])
     return out
 
 
@@ -85,9 +108,9 @@ def build_all(verbose: bool = True) -> list[str]:
     os.makedirs(LIB, exist_ok=True)
     objdir = os.path.join(PKG, "_build")
     os.makedirs(objdir, exist_ok=True)
-    built = [build_runtime(objdir)]
+    built = [build_translator(), build_runtime(objdir)]
     with ThreadPoolExecutor(max_workers=4) as ex:
-        built += list(ex.map(build_program, PROGRAM_NAMES))
+        built += list(ex.map(build_program, PROGRAM_SOURCES))
     shutil.rmtree(objdir, ignore_errors=True)
     if verbose:
         for b in built:

@@ -1,0 +1,134 @@
+// physisc-hip: translation of Physis programs for the HIP target.
+//
+// Responsibilities mirror the reference's target plug-in:
+//   * stencil analysis (translator/stencil_analysis.cc:64-143,
+//     stencil_range.h:136-176): per grid parameter of a stencil function,
+//     the constant offsets of its gets, halo widths and periodicity;
+//   * HIPTranslator (cf. CUDATranslator, cuda_translator.h:12-57): rewrites
+//     PS* intrinsics in host code and emits the program's functions;
+//   * HIPRuntimeBuilder (cf. CUDARuntimeBuilder, cuda_runtime_builder.cc:
+//     1099-1641): emits stencil structs, map functions, the device form of
+//     each stencil function, its run kernel and launcher, and the host run
+//     function of each PSStencilRun call site.
+#ifndef PHYSIS_AMD_TRANSLATOR_TRANSLATOR_H_
+#define PHYSIS_AMD_TRANSLATOR_TRANSLATOR_H_
+
+#include <array>
+#include <map>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "ast.h"
+
+namespace physisc {
+
+struct Config {
+  // Launch shapes of the 2.5D run kernel (waves per block, prefetch depth).
+  int block_waves = 0;  // 0: chosen from the LDS budget
+  int prefetch = 2;
+  bool kernel_error_check = false;  // HIP_KERNEL_ERROR_CHECK
+  bool trace = false;               // TRACE_KERNEL
+};
+
+// One grid parameter of a stencil function.
+struct GridParam {
+  std::string name;
+  int dims = 3;
+  std::string elem;    // C element type: float double int long
+  std::string ps_type; // PS_FLOAT ...
+  std::string dev_view;  // __PSGrid3DFloat_dev ...
+  // analysis
+  bool read = false, emitted = false;
+  std::vector<std::array<int, 3>> offsets;  // regular gets, first-seen order
+  std::vector<bool> offset_periodic;
+  bool irregular = false;   // some get is not index+constant
+  bool periodic = false;    // staged tile wraps
+  int hx = 0, hy = 0, zl = 0, zr = 0;
+  enum Mode { kNone, kStaged, kDirect } mode = kNone;
+};
+
+struct ScalarParam {
+  std::string type, name;
+};
+
+struct KernelInfo {
+  const Function *fn = nullptr;
+  std::string name;
+  int ndims = 3;
+  std::vector<std::string> index;  // x y z parameter names
+  std::vector<GridParam> grids;
+  std::vector<ScalarParam> scalars;
+  // parameter order: 'g' grid i / 's' scalar i
+  std::vector<std::pair<char, int>> order;
+  std::string ret_type = "void";
+  bool fast = false;          // 2.5D template path
+  std::string slow_reason;    // why not
+  int hy = 0;                 // common y halo of staged grids
+  bool used_plain = false, used_reduce = false;
+  int GridIndex(const std::string &n) const {
+    for (size_t i = 0; i < grids.size(); ++i)
+      if (grids[i].name == n) return (int)i;
+    return -1;
+  }
+  const GridParam *Grid(const std::string &n) const {
+    int i = GridIndex(n);
+    return i < 0 ? nullptr : &grids[i];
+  }
+};
+
+// A PSStencilRun call site: the kernels of its maps, in order.
+struct RunSite {
+  int id;
+  std::vector<std::string> kernels;
+};
+
+class Translator {
+ public:
+  Translator(Program &prog, const Config &cfg,
+             const std::vector<std::pair<std::string, std::string>> &globals);
+  std::string Translate();
+
+ private:
+  // analysis
+  void FindKernels();
+  void FindKernelsIn(const Stmt &s);
+  void FindKernelsIn(const Expr &e);
+  void AnalyzeKernel(KernelInfo &k);
+  void AnalyzeGets(KernelInfo &k, const Expr &e);
+  void AnalyzeGets(KernelInfo &k, const Stmt &s);
+  bool RegularIndex(const KernelInfo &k, const Expr &e, int axis, int *off);
+  void Decide(KernelInfo &k);
+
+  // device code
+  void RewriteKernelExpr(const KernelInfo &k, ExprP &e);
+  void RewriteKernelStmt(const KernelInfo &k, Stmt &s);
+  std::string EmitKernel(KernelInfo &k);
+  std::string EmitStencilStruct(const KernelInfo &k);
+  std::string EmitContext(const KernelInfo &k);
+  std::string EmitDeviceFunction(KernelInfo &k);
+  std::string EmitFastRunKernel(const KernelInfo &k);
+  std::string EmitGenericRunKernel(const KernelInfo &k);
+  std::string EmitLauncher(const KernelInfo &k);
+  std::string EmitRunFunction(const RunSite &r);
+
+  // host code
+  void RewriteHostExpr(ExprP &e, const Function &f);
+  void RewriteHostStmt(Stmt &s, const Function &f);
+  std::string GridElemOf(const Expr &g, const Function &f) const;
+  ExprP MapCall(Expr &call, const Function &f);
+
+  Program &prog_;
+  Config cfg_;
+  std::map<std::string, std::string> grid_vars_;  // global name -> type
+  std::map<std::string, std::map<std::string, std::string>> local_grids_;
+  std::map<std::string, KernelInfo> kernels_;
+  std::vector<std::string> kernel_order_;
+  std::set<std::string> device_helpers_;
+  std::vector<RunSite> runs_;
+  std::map<const Function *, std::vector<int>> runs_in_;
+};
+
+}  // namespace physisc
+
+#endif  // PHYSIS_AMD_TRANSLATOR_TRANSLATOR_H_

@@ -1,0 +1,51 @@
+"""The reference's system tests, translated by physisc-hip and run on the
+GPU: each dump must hash to the reference oracle's output
+(tests/golden/ref_outputs.json, the same check tests/system_tests/
+run_system_tests.sh.cmake:833-883 performs with diff)."""
+import ctypes as C
+import json
+
+import numpy as np
+import pytest
+
+from oracle_lib import GOLDEN, dump_f, sha256
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(GOLDEN))["system_tests"]
+
+FLOAT_TESTS = {
+    "test_7-pt": ("systest_7pt", 32 ** 3),
+    "test_7-pt-periodic": ("systest_7pt_periodic", 32 ** 3),
+    "test_7-pt-neumann-cond": ("systest_7pt_neumann", 8 ** 3),
+    "test_7-pt-multi-iterations": ("systest_7pt_multi_iterations", 32 ** 3),
+    "test_asymmetric": ("systest_asymmetric", 32 ** 3),
+    "test_asymmetric-periodic": ("systest_asymmetric_periodic", 32 ** 3),
+    "test_27-pt": ("systest_27pt", 32 ** 3),
+    "test_27-pt-periodic": ("systest_27pt_periodic", 32 ** 3),
+}
+
+
+@pytest.fixture(scope="module")
+def prog(ps):
+    return ps.load_program("systests")
+
+
+@pytest.mark.parametrize("name", sorted(FLOAT_TESTS))
+def test_float_dump_matches_reference(prog, name):
+    fn, n = FLOAT_TESTS[name]
+    out = np.zeros(n, np.float32)
+    getattr(prog, fn)(out.ctypes.data_as(C.c_void_p))
+    assert sha256(dump_f(out)) == GOLD[name]["sha256"], name
+
+
+def test_27pt_reduction_matches_reference(prog):
+    v1, v2 = C.c_int(), C.c_int()
+    prog.systest_27pt_reduction(C.byref(v1), C.byref(v2))
+    assert [str(v1.value), str(v2.value)] == GOLD["test_27-pt-reduction"]["values"]
+
+
+def test_reduction_3d_sum_matches_reference(prog):
+    v = C.c_float()
+    prog.systest_reduction_3d_sum(C.byref(v))
+    assert sha256(("%f\n" % v.value).encode()) == \
+        GOLD["test_reduction-3d-sum"]["sha256"]

@@ -1,0 +1,89 @@
+"""physisc-hip on the CPU: the committed translations are what the
+translator produces, the stencil analysis finds the reference's offsets,
+and unsupported constructs are rejected with an error."""
+import os
+import subprocess
+import tempfile
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PHYSISC = os.path.join(ROOT, "physis_amd", "bin", "physisc-hip")
+SOURCES = [os.path.join(ROOT, "physis_amd", "programs", "diffusion7.c"),
+           os.path.join(ROOT, "physis_amd", "programs", "himeno.c"),
+           os.path.join(ROOT, "tests", "programs", "systests.c")]
+
+
+def translate(src, text=None):
+    with tempfile.TemporaryDirectory() as d:
+        if text is not None:
+            src = os.path.join(d, "prog.c")
+            open(src, "w").write(text)
+        out = os.path.join(d, "out.hip.cpp")
+        p = subprocess.run([PHYSISC, src, "-o", out], capture_output=True,
+                           text=True)
+        return p.returncode, (open(out).read() if p.returncode == 0 else ""), \
+            p.stderr
+
+
+@pytest.mark.parametrize("src", SOURCES)
+def test_committed_translation_is_current(src):
+    rc, code, err = translate(src)
+    assert rc == 0, err
+    committed = open(src[:-2] + ".hip.cpp").read()
+    assert code == committed, f"re-run physisc-hip on {src}"
+
+
+def test_himeno_analysis_finds_19_points():
+    _, code, _ = translate(SOURCES[1])
+    assert "p0  staged x[-1,1] y[-1,1] z[-1,1] non-periodic" in code
+    assert " 18:(0,0,0)" in code
+    for g in ("a0", "a1", "a2", "a3", "b0", "b1", "b2", "c0", "c1", "c2",
+              "bnd", "wrk1"):
+        assert f"//   {g}  direct (centre only)" in code
+
+
+PROG = """#include "physis/physis.h"
+static void k(const int x, const int y, const int z, PSGrid3DFloat a,
+              PSGrid3DFloat b, int s) {
+  PSGridEmit(b, PSGridGet(a, x, y, %s) + PSGridGetPeriodic(a, x+3, y, z));
+}
+void run(PSGrid3DFloat a, PSGrid3DFloat b) {
+  PSDomain3D d = PSDomain3DNew(0, 8, 0, 8, 0, 8);
+  PSStencilRun(PSStencilMap(k, d, a, b, 1), 3);
+}
+"""
+
+
+def test_irregular_get_reads_memory():
+    rc, code, err = translate(None, PROG % "s")
+    assert rc == 0, err
+    assert "physis_hip::GetLinear<float>" in code
+    assert "x[-3,3]" in code and "periodic" in code
+
+
+def test_two_dimensional_kernel_uses_generic_path():
+    src = """#include "physis/physis.h"
+static void k(const int x, const int y, PSGrid2DFloat a, PSGrid2DFloat b) {
+  PSGridEmit(b, PSGridGet(a, x+1, y) + PSGridGet(a, x-1, y));
+}
+void run(PSGrid2DFloat a, PSGrid2DFloat b) {
+  PSDomain2D d = PSDomain2DNew(1, 7, 0, 8);
+  PSStencilRun(PSStencilMap(k, d, a, b));
+}
+"""
+    rc, code, err = translate(None, src)
+    assert rc == 0, err
+    assert "__PSStencilRunGeneric_k" in code
+
+
+def test_red_black_maps_are_rejected():
+    src = PROG.replace("PSStencilMap(k", "PSStencilMapRedBlack(k") % "z"
+    rc, _, err = translate(None, src)
+    assert rc != 0 and "red-black" in err
+
+
+def test_missing_kernel_is_an_error():
+    src = PROG.replace("PSStencilMap(k,", "PSStencilMap(nokernel,") % "z"
+    rc, _, err = translate(None, src)
+    assert rc != 0 and "nokernel" in err
