@@ -131,9 +131,9 @@ def main() -> int:
                     help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} "
-              "is not supported (the multi-GPU z-slab design is in DESIGN.md)",
-              file=sys.stderr)
+        print(f"bench.py: measures one GPU only; --gpus {args.gpus} is not "
+              "supported (a multi-GPU run would shard the grid into z-slabs "
+              "with one halo plane per neighbour)", file=sys.stderr)
         return 2
 
     from physis_amd import runtime as ps

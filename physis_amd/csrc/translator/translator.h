@@ -1,6 +1,6 @@
 // physisc-hip: translation of Physis programs for the HIP target.
 //
-// Responsibilities mirror the reference's target plug-in (SURVEY §8b):
+// Responsibilities mirror the reference's target plug-in:
 //   * stencil analysis (translator/stencil_analysis.cc:64-143,
 //     stencil_range.h:136-176): per grid parameter of a stencil function,
 //     the constant offsets of its gets, halo widths and periodicity;

@@ -1,0 +1,61 @@
+"""Workload for rocprofv3 measurements of the hot-path kernels.
+
+Run under the profiler, e.g.
+    rocprofv3 --pmc FETCH_SIZE -d OUT -o fetch --output-format csv -- \\
+        python3 -m physis_amd.profile_kernels
+It issues, in order: a device-to-device copy of one 512 MiB grid (the
+runtime's plain copy kernel, whose FETCH_SIZE / WRITE_SIZE calibrate the
+counters' units), 10 sweeps of the 7-point clamped and periodic run kernels
+at 512^3, 10 Himeno sweeps at 512x256x256, and 3 PSReduce SUMs at 512^3.
+Then `python3 -m physis_amd.profile_summary OUT_fetch OUT_write` writes the
+per-kernel HBM traffic summary.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import runtime as ps
+from .configs import DIFFUSION_COEF, diffusion_input, himeno_inputs
+from .programs import Diffusion7, Himeno
+
+COPY_BYTES = 512 * 2 ** 20
+
+
+def main() -> None:
+    ps.PSInit()
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpyDtoD.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+    hip.hipDeviceSynchronize.restype = C.c_int
+    shape = (512, 512, 512)
+    a = diffusion_input(*shape, seed=0)
+    g1, g2 = ps.PSGridNew(shape), ps.PSGridNew(shape)
+    ps.PSGridCopyin(g1, a)
+    for _ in range(3):
+        assert hip.hipMemcpyDtoD(g2.handle.contents.p, g1.handle.contents.p,
+                                 COPY_BYTES) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    d7 = Diffusion7()
+    d7.run(g1, g2, 10, DIFFUSION_COEF)
+    d7.run(g1, g2, 10, DIFFUSION_COEF, periodic=True)
+    for _ in range(3):
+        ps.PSReduce(ps.PS_SUM, g1)
+    ps.PSGridFree(g1)
+    ps.PSGridFree(g2)
+    hs = (512, 256, 256)
+    coefs, p = himeno_inputs(*hs, seed=0)
+    grids = []
+    for c in coefs + [p, p]:
+        g = ps.PSGridNew(hs)
+        ps.PSGridCopyin(g, c)
+        grids.append(g)
+    Himeno().run(10, grids[12], grids[13], grids[:12])
+    for g in grids:
+        ps.PSGridFree(g)
+    ps.PSFinalize()
+    print("profile workload done", float(np.float32(0)))
+
+
+if __name__ == "__main__":
+    main()

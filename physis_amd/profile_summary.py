@@ -1,0 +1,147 @@
+"""Summarises rocprofv3 output of physis_amd.profile_kernels.
+
+    python3 -m physis_amd.profile_summary FETCH_DIR WRITE_DIR KTRACE_DIR OUT.md
+
+FETCH_DIR / WRITE_DIR hold the *_counter_collection.csv of a
+`--pmc FETCH_SIZE` and a `--pmc WRITE_SIZE` run, KTRACE_DIR the
+*_kernel_stats.csv of a `--kernel-trace --stats` run of the same workload.
+Counter values are converted to bytes with the factor that makes the
+512 MiB device-to-device copy read and write exactly 512 MiB.  Writes a
+Markdown table (OUT.md) and, next to it, diffusion7_512_traffic.json with the
+7-point run kernel's HBM bytes per launch (read by bench.py).
+"""
+from __future__ import annotations
+
+import csv
+import glob
+import json
+import os
+import sys
+from collections import defaultdict
+
+COPY_BYTES = 512 * 2 ** 20
+KERNELS = [  # (label, name fragment, algorithmic bytes per launch)
+    ("7-pt clamped run kernel, 512^3", "__PSStencilRun_diffusion7_kernel<",
+     8 * 512 ** 3),
+    ("7-pt periodic run kernel, 512^3",
+     "__PSStencilRun_diffusion7_periodic_kernel<", 8 * 512 ** 3),
+    ("19-pt Himeno run kernel, 512x256x256", "__PSStencilRun_jacobi_kernel<",
+     56 * 512 * 256 * 256),
+    ("PSReduce SUM stage 1, 512^3", "ReduceStage1<float, 2>", 4 * 512 ** 3),
+]
+
+
+def _counter(d: str, name: str):
+    f = glob.glob(os.path.join(d, "*_counter_collection.csv"))[0]
+    vals = defaultdict(list)
+    for r in csv.DictReader(open(f)):
+        if r["Counter_Name"] == name:
+            vals[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return vals
+
+
+def _pick(vals, frag):
+    ks = [k for k in vals if frag in k]
+    if len(ks) != 1:
+        raise SystemExit(f"kernel {frag!r}: {len(ks)} matches")
+    v = vals[ks[0]]
+    return ks[0], sum(v) / len(v), len(v)
+
+
+def main(fetch_dir, write_dir, kt_dir, out_md) -> None:
+    fetch = _counter(fetch_dir, "FETCH_SIZE")
+    write = _counter(write_dir, "WRITE_SIZE")
+    copy = [k for k in write if "copyBuffer" in k][0]
+    big_w = sorted(write[copy])[-3:]
+    big_f = sorted(fetch[copy])[-3:]
+    ws = COPY_BYTES / (sum(big_w) / len(big_w))
+    fs = COPY_BYTES / (sum(big_f) / len(big_f))
+    kt = {}
+    for f in glob.glob(os.path.join(kt_dir, "*_kernel_stats.csv")):
+        for r in csv.DictReader(open(f)):
+            kt[r["Name"]] = (float(r["AverageNs"]), int(r["Calls"]))
+    rows = []
+    traffic7 = None
+    for label, frag, alg in KERNELS:
+        name, f, n = _pick(fetch, frag)
+        _, w, _ = _pick(write, frag)
+        fb, wb = f * fs, w * ws
+        cands = [(k, v) for k, v in kt.items() if frag in k]
+        if len(cands) > 1:  # the plain (non-reducing) VX = 4 instantiation
+            cands = [(k, v) for k, v in cands if "<4" in k and "-1>" in k]
+        avg_ns = cands[0][1][0] if cands else float("nan")
+        rows.append((label, fb, wb, alg, avg_ns, n))
+        if "diffusion7_kernel<" in frag:
+            traffic7 = fb + wb
+    lines = [
+        "# HBM traffic of the hot-path kernels (MI355X)",
+        "",
+        "Workload: `python3 -m physis_amd.profile_kernels`. Commands:",
+        "",
+        "```",
+        "rocprofv3 --kernel-trace --stats -d OUT/kt -o kt --output-format csv "
+        "-- python3 -m physis_amd.profile_kernels",
+        "rocprofv3 --pmc FETCH_SIZE -d OUT/fetch -o fetch --output-format csv "
+        "-- python3 -m physis_amd.profile_kernels",
+        "rocprofv3 --pmc WRITE_SIZE -d OUT/write -o write --output-format csv "
+        "-- python3 -m physis_amd.profile_kernels",
+        "python3 -m physis_amd.profile_summary OUT/fetch OUT/write OUT/kt "
+        "profiles/hot_kernels_traffic.md",
+        "```",
+        "",
+        f"Counter units: the 512 MiB device-to-device copy reads "
+        f"{sum(big_f)/len(big_f):.0f} FETCH_SIZE units and writes "
+        f"{sum(big_w)/len(big_w):.0f} WRITE_SIZE units, so 1 FETCH_SIZE unit "
+        f"= {fs:.1f} B and 1 WRITE_SIZE unit = {ws:.1f} B here.",
+        "",
+        "| kernel | HBM read MB | HBM write MB | algorithmic MB | traffic / "
+        "algorithmic | avg time (us) | achieved algorithmic GB/s | % of 8 TB/s |",
+        "|---|---|---|---|---|---|---|---|",
+    ]
+    for label, fb, wb, alg, ns, n in rows:
+        gbs = alg / (ns * 1e-9) / 1e9 if ns == ns else float("nan")
+        lines.append(f"| {label} | {fb/1e6:.1f} | {wb/1e6:.1f} | {alg/1e6:.1f} "
+                     f"| {(fb+wb)/alg:.3f} | {ns/1e3:.1f} | {gbs:.0f} | "
+                     f"{gbs/8000*100:.1f} |")
+    lines.append("")
+    os.makedirs(os.path.dirname(os.path.abspath(out_md)), exist_ok=True)
+    open(out_md, "w").write("\n".join(lines))
+    jpath = os.path.join(os.path.dirname(os.path.abspath(out_md)),
+                         "diffusion7_512_traffic.json")
+    json.dump({"kernel": "__PSStencilRun_diffusion7_kernel<4, 16, -1>",
+               "hbm_bytes_per_launch": int(traffic7),
+               "algorithmic_bytes_per_launch": 8 * 512 ** 3,
+               "fetch_bytes_per_unit": fs, "write_bytes_per_unit": ws,
+               "source": os.path.basename(out_md)},
+              open(jpath, "w"), indent=1)
+    print("\n".join(lines))
+
+
+def ktrace_md(kt_dir: str, out_md: str, command: str) -> None:
+    """Kernel-trace statistics of any command as a Markdown table."""
+    rows = []
+    for f in glob.glob(os.path.join(kt_dir, "*_kernel_stats.csv")):
+        for r in csv.DictReader(open(f)):
+            rows.append((r["Name"], int(r["Calls"]), float(r["AverageNs"]),
+                         float(r["MinNs"]), float(r["MaxNs"]),
+                         float(r["Percentage"])))
+    lines = ["# Kernel trace (rocprofv3 --kernel-trace --stats, MI355X)", "",
+             "```", command, "```", "",
+             "| kernel | calls | avg us | min us | max us | % time |",
+             "|---|---|---|---|---|---|"]
+    for n, c, a, lo, hi, pct in sorted(rows, key=lambda r: -r[5]):
+        name = n.split("(")[0].replace("|", "/")
+        lines.append(f"| `{name}` | {c} | {a/1e3:.1f} | {lo/1e3:.1f} | "
+                     f"{hi/1e3:.1f} | {pct:.2f} |")
+    lines.append("")
+    open(out_md, "w").write("\n".join(lines))
+    print("\n".join(lines))
+
+
+if __name__ == "__main__":
+    if len(sys.argv) == 5 and sys.argv[1] == "--ktrace":
+        ktrace_md(*sys.argv[2:])
+    elif len(sys.argv) == 5:
+        main(*sys.argv[1:])
+    else:
+        raise SystemExit(__doc__)
