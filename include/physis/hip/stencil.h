@@ -50,6 +50,37 @@ template <> struct NativeVec<long, 2> {
   typedef long V __attribute__((ext_vector_type(2)));
 };
 
+// ------------------------------------------------ 16-bit storage formats
+// Grids of the precision tier hold bfloat16 or IEEE binary16 values as raw
+// bit patterns; arithmetic converts them to float.  float -> 16-bit rounds to
+// nearest even.
+struct bf16_t { unsigned short bits; };
+struct fp16_t { unsigned short bits; };
+
+template <typename To, typename From>
+__device__ __forceinline__ To Convert(From v) { return (To)v; }
+template <> __device__ __forceinline__ float Convert<float, bf16_t>(bf16_t v) {
+  return __uint_as_float((unsigned)v.bits << 16);
+}
+template <> __device__ __forceinline__ double Convert<double, bf16_t>(bf16_t v) {
+  return (double)__uint_as_float((unsigned)v.bits << 16);
+}
+template <> __device__ __forceinline__ float Convert<float, fp16_t>(fp16_t v) {
+  return (float)__builtin_bit_cast(_Float16, v.bits);
+}
+template <> __device__ __forceinline__ double Convert<double, fp16_t>(fp16_t v) {
+  return (double)(float)__builtin_bit_cast(_Float16, v.bits);
+}
+template <> __device__ __forceinline__ bf16_t Convert<bf16_t, float>(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return bf16_t{(unsigned short)0x7fc0};
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return bf16_t{(unsigned short)(u >> 16)};
+}
+template <> __device__ __forceinline__ fp16_t Convert<fp16_t, float>(float f) {
+  return fp16_t{__builtin_bit_cast(unsigned short, (_Float16)f)};
+}
+
 // True when N consecutive T can move as one native vector.
 template <typename T, int N> struct CanVec {
   static constexpr bool value =
@@ -113,6 +144,9 @@ struct BufWin {
     } else if constexpr (kBytes == 4) {
       auto v = __builtin_amdgcn_raw_buffer_load_b32(rsrc, b, 0, 0);
       __builtin_memcpy(out, &v, 4);
+    } else if constexpr (kBytes == 2) {
+      auto v = __builtin_amdgcn_raw_buffer_load_b16(rsrc, b, 0, 0);
+      __builtin_memcpy(out, &v, 2);
     } else {
 #pragma unroll
       for (int i = 0; i < N; ++i) {
@@ -148,6 +182,10 @@ struct BufWin {
       unsigned w;
       __builtin_memcpy(&w, v, 4);
       __builtin_amdgcn_raw_buffer_store_b32(w, rsrc, b, 0, AUX);
+    } else if constexpr (kBytes == 2) {
+      unsigned short w;
+      __builtin_memcpy(&w, v, 2);
+      __builtin_amdgcn_raw_buffer_store_b16(w, rsrc, b, 0, AUX);
     } else {
 #pragma unroll
       for (int i = 0; i < N; ++i) {
@@ -201,8 +239,10 @@ __device__ __forceinline__ TileCoord TileOf(int tiles_x, int tiles_y) {
 // halo and the other lanes repeat lane 0's address.
 //   HX, HY  halo width in x and y (maximum |offset| the stencil uses)
 //   NSLOT   planes in the LDS ring (ZR - ZL + 2)
+//   M       element type in global memory; values are converted to T as
+//           they enter LDS (T = M for Physis grids)
 template <typename T, int VX, int BY, int HX, int HY, int NSLOT,
-          bool PERIODIC>
+          bool PERIODIC, typename M = T>
 struct Stage {
   static constexpr int BX = kWave;
   static constexpr int kTX = BX * VX;
@@ -216,8 +256,8 @@ struct Stage {
   static constexpr int kSlots = NSLOT;
   static constexpr int kLdsElems = kPlane * NSLOT;
 
-  T own[VX];
-  T halo;
+  M own[VX];
+  M halo;
 
   // Linear offset of (x, y, z): wrapped per axis when periodic.
   __device__ __forceinline__ static int Offset(const int *dim, int x, int y,
@@ -234,17 +274,17 @@ struct Stage {
   // Non-periodic reads go through the block's window w (the REF linear rule
   // may spill into the neighbouring plane); periodic reads go through a
   // window of exactly the wrapped plane, which may lie anywhere in the grid.
-  __device__ __forceinline__ void Load(const BufWin<T> &w, const T *p,
+  __device__ __forceinline__ void Load(const BufWin<M> &w, const M *p,
                                        const int *dim, int x0, int y, int z,
                                        int lane) {
     const int hl = lane < 2 * HX ? lane : 0;
     const int hx = hl < HX ? x0 - HX + hl : x0 + kTX + (hl - HX);
-    T v[1];
+    M v[1];
     if (PERIODIC) {
       const int nxy = dim[0] * dim[1];
       const int zw = WrapIndex(z, dim[2]);
-      const BufWin<T> pw =
-          BufWin<T>::Make(p, nxy * dim[2], (long)zw * nxy, (long)(zw + 1) * nxy);
+      const BufWin<M> pw =
+          BufWin<M>::Make(p, nxy * dim[2], (long)zw * nxy, (long)(zw + 1) * nxy);
       const int yw = WrapIndex(y, dim[1]) * dim[0] + zw * nxy;
       pw.template Load<VX>(WrapIndex(x0 + lane * VX, dim[0]) + yw, own);
       pw.template Load<1>(WrapIndex(hx, dim[0]) + yw, v);
@@ -258,9 +298,17 @@ struct Stage {
   // Writes the wave's row of the plane into one LDS slot.
   __device__ __forceinline__ void Store(T *slot, int wave, int lane) const {
     T *row = slot + wave * kRow;
-    StoreVec(row + kPad + lane * VX, own);
+    if constexpr (__is_same(T, M)) {
+      StoreVec(row + kPad + lane * VX, own);
+    } else {
+      T cv[VX];
+#pragma unroll
+      for (int i = 0; i < VX; ++i) cv[i] = Convert<T, M>(own[i]);
+      StoreVec(row + kPad + lane * VX, cv);
+    }
     if (lane < 2 * HX)
-      row[lane < HX ? kPad - HX + lane : kPad + kTX + (lane - HX)] = halo;
+      row[lane < HX ? kPad - HX + lane : kPad + kTX + (lane - HX)] =
+          Convert<T, M>(halo);
   }
 
   // Row segment of `slot` at row `row` covering x offsets XL .. VX-1+XR

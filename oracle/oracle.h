@@ -103,6 +103,33 @@ void oracle_test_asymmetric_periodic(const float *in, float *out, int n); /* tes
 void oracle_test_27pt(const float *in, float *out, int n, int periodic);  /* test_27-pt{,-periodic}.c */
 void oracle_test_27pt_int(const int *in, int *out, int n);          /* test_27-pt-reduction.c:14-40 */
 
+/* ---- Poisson solver extensions (oracle/poisson.c).  storage: 0 FP32,
+ * 1 bfloat16, 2 binary16 (16-bit values as uint16 bit patterns).  Grids hold
+ * nbatch problems of nz/nbatch planes stacked along z; faces of every problem
+ * keep their values.  Coefficient grids: a0 a1 a2 a3 b0 b1 b2 c0 c1 c2 bnd
+ * wrk1 (the sweep's last field is its right-hand side). */
+uint16_t oracle_f32_to_bf16(float f);
+float oracle_bf16_to_f32(uint16_t h);
+uint16_t oracle_f32_to_f16(float f);
+float oracle_f16_to_f32(uint16_t h);
+void oracle_to_storage(int storage, const float *in, void *out, int64_t n);
+void oracle_poisson_sweep(int storage, const void *pin, void *pout,
+                          const void *const *f, int nx, int ny, int nz,
+                          int nbatch, float omega);
+void oracle_poisson_residual(int storage, const void *p, int p64,
+                             double *p_out, const void *e, double inv_sigma,
+                             double sigma, void *rhs, const float *const *cf,
+                             int nx, int ny, int nz, int nbatch,
+                             double *sums);
+void oracle_poisson_solve_mixed(int storage, const float *const *cf,
+                                double *p, int nx, int ny, int nz, int nbatch,
+                                float omega, double tol, int inner,
+                                long max_sweeps, double *stats);
+double oracle_poisson_jacobi_fp64(const float *const *cf, double *p, int nx,
+                                  int ny, int nz, int nbatch, float omega,
+                                  double tol, int check, long max_sweeps,
+                                  long *sweeps_out);
+
 #ifdef __cplusplus
 }
 #endif

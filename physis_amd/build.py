@@ -39,6 +39,10 @@ RUNTIME_SOURCES = [
     os.path.join(CSRC, "runtime", "buffer_hip.hip"),
 ]
 
+# Poisson solver extensions (precision tier, plain FP32 solve) over the
+# runtime's grids: libphysis_poisson_hip.
+POISSON_SOURCES = [os.path.join(CSRC, "poisson", "mixed_hip.hip")]
+
 TRANSLATOR_SOURCES = [os.path.join(CSRC, "translator", f) for f in
                       ("lexer.cc", "parser.cc", "translator.cc", "physisc.cc")]
 PHYSISC = os.path.join(PKG, "bin", "physisc-hip")
@@ -82,6 +86,18 @@ def build_runtime(objdir: str) -> str:
     return out
 
 
+def build_poisson(objdir: str) -> str:
+    objs = []
+    for src in POISSON_SOURCES:
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        _compile_object(src, obj)
+        objs.append(obj)
+    out = os.path.join(LIB, "libphysis_poisson_hip.so")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o",
+          out, f"-L{LIB}", "-lphysis_rt_hip", "-Wl,-rpath,$ORIGIN"])
+    return out
+
+
 def build_translator() -> str:
     os.makedirs(os.path.dirname(PHYSISC), exist_ok=True)
     _run(["g++", "-std=c++17", "-O2", "-Wall", *TRANSLATOR_SOURCES, "-o",
@@ -111,7 +127,9 @@ def build_all(verbose: bool = True) -> list[str]:
     os.makedirs(objdir, exist_ok=True)
     built = [build_translator(), build_runtime(objdir)]
     with ThreadPoolExecutor(max_workers=4) as ex:
+        poisson = ex.submit(build_poisson, objdir)
         built += list(ex.map(build_program, PROGRAM_SOURCES))
+        built.append(poisson.result())
     shutil.rmtree(objdir, ignore_errors=True)
     if verbose:
         for b in built:

@@ -52,3 +52,73 @@ def himeno_inputs(nx: int, ny: int, nz: int, seed: int = 0):
     p = (p + np.float32(1e-3) * uniform01(n, seed=seed, start=10 * n)) \
         .astype(np.float32)
     return coefs, p
+
+
+def gaussian_rhs(nx: int, ny: int, nz: int, count: int, sig_lo: float,
+                 sig_hi: float, seed: int = 0, stream: int = 0) -> np.ndarray:
+    """float32[nz*ny*nx]: a sum of `count` 3-D Gaussians
+    a * exp(-|r - c|^2 / (2 s^2)) with centre c uniform over the grid,
+    s uniform in [sig_lo, sig_hi] cells and amplitude a = +1 or -1 (the
+    reading of "amplitudes +-1"), all drawn from the seeded stream (five
+    draws per Gaussian, starting at draw 5*count*stream).  Evaluated in FP64
+    as separable products, rounded once to float32."""
+    u = uniform01(5 * count, seed=seed, start=5 * count * stream) \
+        .astype(np.float64)
+    x = np.arange(nx, dtype=np.float64)
+    y = np.arange(ny, dtype=np.float64)
+    z = np.arange(nz, dtype=np.float64)
+    out = np.zeros((nz, ny, nx), np.float64)
+    for g in range(count):
+        cx, cy, cz = u[5 * g] * nx, u[5 * g + 1] * ny, u[5 * g + 2] * nz
+        s = sig_lo + (sig_hi - sig_lo) * u[5 * g + 3]
+        a = 1.0 if u[5 * g + 4] >= 0.5 else -1.0
+        k = -0.5 / (s * s)
+        gx = np.exp(k * (x - cx) ** 2)
+        gy = np.exp(k * (y - cy) ** 2)
+        gz = np.exp(k * (z - cz) ** 2)
+        out += a * gz[:, None, None] * (gy[:, None] * gx[None, :])[None]
+    return out.astype(np.float32).reshape(-1)
+
+
+def poisson_problem(nx: int, ny: int, nzs: int, nbatch: int = 1,
+                    noise: float = 0.01, gaussians=(16, 16),
+                    sigma=(4.0, 32.0), seed: int = 0):
+    """Coefficient grids (a0 a1 a2 a3 b0 b1 b2 c0 c1 c2 bnd wrk1) of nbatch
+    Poisson problems of nx x ny x nzs stacked along z.
+
+    The operator is the Himeno one of himeno_inputs with relative noise
+    `noise` (b additive); bnd = 0 on the faces of every problem; wrk1 of
+    problem q is a sum of gaussians[0] .. gaussians[1] Gaussians (the count
+    drawn per problem) with widths in `sigma`, masked by bnd.
+    """
+    nz = nzs * nbatch
+    n = nx * ny * nz
+    one, pct = np.float32(1.0), np.float32(noise)
+
+    def nz_(k):
+        u = uniform01(n, seed=seed, start=k * n)
+        return (np.float32(2.0) * u - one).astype(np.float32)
+
+    coefs = []
+    for k in range(3):
+        coefs.append((one * (one + pct * nz_(k))).astype(np.float32))
+    coefs.append((np.float32(1.0 / 6.0) * (one + pct * nz_(3)))
+                 .astype(np.float32))
+    for k in range(3):
+        coefs.append((pct * nz_(4 + k)).astype(np.float32))
+    for k in range(3):
+        coefs.append((one * (one + pct * nz_(7 + k))).astype(np.float32))
+    bnd = np.zeros((nbatch, nzs, ny, nx), np.float32)
+    bnd[:, 1:-1, 1:-1, 1:-1] = 1.0
+    bnd = bnd.reshape(-1)
+    coefs.append(bnd)
+    lo, hi = gaussians
+    counts = uniform01(nbatch, seed=seed + 1)
+    rhs = np.empty(n, np.float32)
+    m = nx * ny * nzs
+    for q in range(nbatch):
+        c = lo + min(int(counts[q] * (hi - lo + 1)), hi - lo)
+        rhs[q * m:(q + 1) * m] = gaussian_rhs(nx, ny, nzs, c, *sigma,
+                                              seed=seed + 2, stream=q)
+    coefs.append((rhs * bnd).astype(np.float32))
+    return coefs

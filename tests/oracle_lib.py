@@ -73,8 +73,105 @@ def lib() -> C.CDLL:
         L.oracle_diffusion_coefs.argtypes = [C.c_int, C.c_int, C.c_int, FP]
         L.oracle_diffusion_init.argtypes = [FP, C.c_int, C.c_int, C.c_int,
                                             C.c_int]
+        V = C.c_void_p
+        L.oracle_f32_to_bf16.argtypes = [C.c_float]
+        L.oracle_f32_to_bf16.restype = C.c_uint16
+        L.oracle_bf16_to_f32.argtypes = [C.c_uint16]
+        L.oracle_bf16_to_f32.restype = C.c_float
+        L.oracle_f32_to_f16.argtypes = [C.c_float]
+        L.oracle_f32_to_f16.restype = C.c_uint16
+        L.oracle_f16_to_f32.argtypes = [C.c_uint16]
+        L.oracle_f16_to_f32.restype = C.c_float
+        L.oracle_to_storage.argtypes = [C.c_int, FP, V, C.c_int64]
+        L.oracle_poisson_sweep.argtypes = [C.c_int, V, V, V, C.c_int, C.c_int,
+                                           C.c_int, C.c_int, F]
+        L.oracle_poisson_residual.argtypes = [C.c_int, V, C.c_int, V, V,
+                                              C.c_double, C.c_double, V, V,
+                                              C.c_int, C.c_int, C.c_int,
+                                              C.c_int, V]
+        L.oracle_poisson_solve_mixed.argtypes = [C.c_int, V, V, C.c_int,
+                                                 C.c_int, C.c_int, C.c_int, F,
+                                                 C.c_double, C.c_int, C.c_long,
+                                                 V]
+        L.oracle_poisson_jacobi_fp64.argtypes = [V, V, C.c_int, C.c_int,
+                                                 C.c_int, C.c_int, F,
+                                                 C.c_double, C.c_int, C.c_long,
+                                                 C.POINTER(C.c_long)]
+        L.oracle_poisson_jacobi_fp64.restype = C.c_double
         _lib = L
     return _lib
+
+
+# ------------------------------------------------ Poisson solver extensions
+STORAGE_DTYPE = {0: np.float32, 1: np.uint16, 2: np.uint16}
+
+
+def ptrs(arrays):
+    return (C.c_void_p * len(arrays))(*[a.ctypes.data for a in arrays])
+
+
+def to_storage(storage: int, a: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(a, np.float32).reshape(-1)
+    out = np.empty(a.size, STORAGE_DTYPE[storage])
+    lib().oracle_to_storage(storage, a, out.ctypes.data, a.size)
+    return out
+
+
+def from_storage(storage: int, a: np.ndarray) -> np.ndarray:
+    if storage == 0:
+        return a.astype(np.float32)
+    if storage == 2:
+        return a.view(np.float16).astype(np.float32)
+    return (a.astype(np.uint32) << 16).view(np.float32)
+
+
+def poisson_sweep(storage, pin, fields, shape, nbatch, omega=0.8):
+    """One storage-type sweep; fields: 12 storage arrays (last = rhs)."""
+    nx, ny, nz = shape
+    out = np.empty_like(pin)
+    lib().oracle_poisson_sweep(storage, pin.ctypes.data, out.ctypes.data,
+                               ptrs(fields), nx, ny, nz, nbatch, omega)
+    return out
+
+
+def poisson_residual(storage, p, coefs, shape, nbatch, e=None,
+                     inv_sigma=0.0, sigma=0.0):
+    """(p_out or None, rhs or None, sums[nbatch, 3])."""
+    nx, ny, nz = shape
+    p64 = int(p.dtype == np.float64)
+    p_out = np.empty(p.size, np.float64) if e is not None else None
+    rhs = np.zeros(p.size, STORAGE_DTYPE[storage]) if sigma != 0.0 else None
+    sums = np.zeros(3 * nbatch, np.float64)
+    lib().oracle_poisson_residual(
+        storage, p.ctypes.data, p64,
+        p_out.ctypes.data if p_out is not None else None,
+        e.ctypes.data if e is not None else None, inv_sigma, sigma,
+        rhs.ctypes.data if rhs is not None else None, ptrs(coefs), nx, ny, nz,
+        nbatch, sums.ctypes.data)
+    return p_out, rhs, sums.reshape(nbatch, 3)
+
+
+def poisson_solve_mixed(storage, coefs, p, shape, nbatch, omega, tol, inner,
+                        max_sweeps):
+    nx, ny, nz = shape
+    p = np.ascontiguousarray(p, np.float64).copy()
+    stats = np.zeros(4, np.float64)
+    lib().oracle_poisson_solve_mixed(storage, ptrs(coefs), p.ctypes.data, nx,
+                                     ny, nz, nbatch, omega, tol, inner,
+                                     max_sweeps, stats.ctypes.data)
+    return p, {"outer": int(stats[0]), "sweeps": int(stats[1]),
+               "rel_residual": stats[2], "stagnated": int(stats[3])}
+
+
+def poisson_jacobi_fp64(coefs, p, shape, nbatch, omega=0.8, tol=1e-12,
+                        check=50, max_sweeps=10 ** 6):
+    nx, ny, nz = shape
+    p = np.ascontiguousarray(p, np.float64).copy()
+    sw = C.c_long(0)
+    rel = lib().oracle_poisson_jacobi_fp64(ptrs(coefs), p.ctypes.data, nx, ny,
+                                           nz, nbatch, omega, tol, check,
+                                           max_sweeps, C.byref(sw))
+    return p, rel, sw.value
 
 
 def coef_array(coefs):

@@ -45,3 +45,15 @@ def test_programs_export_their_functions(prog, funcs):
     lib = C.CDLL(os.path.join(LIB, f"libphysis_prog_{prog}.so"))
     for f in funcs:
         assert hasattr(lib, f)
+
+
+def test_poisson_library_exports_its_header():
+    text = open(os.path.join(ROOT, "include", "physis",
+                             "physis_poisson.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = sorted(set(re.findall(r"\b(PSPoisson\w+)\s*\(", text)))
+    assert "PSPoissonSolveMixed" in names and "PSPoissonNew" in names
+    C.CDLL(os.path.join(LIB, "libphysis_rt_hip.so"), mode=C.RTLD_GLOBAL)
+    lib = C.CDLL(os.path.join(LIB, "libphysis_poisson_hip.so"))
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
