@@ -122,11 +122,114 @@ def load_traffic():
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def precision_tier(ps, hip) -> dict:
+    """Per-sweep rates of the precision tier at 512^3 (the 19-point operator
+    with 16 Gaussians on the right-hand side) and time to relative residual
+    1e-6 (FP64-verified) of one 128^3 problem: plain FP32 Jacobi against the
+    BF16 / FP16 refinement."""
+    from physis_amd.configs import poisson_problem
+    from physis_amd.poisson import PoissonSolver
+    out = {}
+    log("generating the 512^3 Poisson problem")
+    coefs = poisson_problem(N7, N7, N7, nbatch=1, noise=0.01,
+                            gaussians=(16, 16), sigma=(4.0, 32.0), seed=0)
+    shape = (N7, N7, N7)
+    npts = N7 ** 3
+    grids = []
+    for c in coefs:
+        g = ps.PSGridNew(shape)
+        ps.PSGridCopyin(g, c)
+        grids.append(g)
+    del coefs
+    rates = {}
+    for st, bpp in (("bf16", 28), ("fp16", 28)):
+        s = PoissonSolver(grids, 1, st)
+        s.inner_sweeps(10)
+        e0, e1 = hip.event(), hip.event()
+        hip.record(e0)
+        s.inner_sweeps(100)
+        hip.record(e1)
+        ms = hip.ms(e0, e1) / 100
+        rates[f"inner_sweep_{st}"] = {
+            "ms": round(ms, 5), "Gpts_s": round(npts / ms / 1e6, 2),
+            "bytes_per_pt": bpp,
+            "hbm_frac": round(bpp * npts / ms / 1e6 / HBM_PEAK_GBS, 4)}
+        if st == "bf16":
+            p = ps.PSGridNew(shape, np.float64)
+            q = ps.PSGridNew(shape, np.float64)
+            s.residual(p, q, 0.5, 1.0)
+            t0 = time.perf_counter()
+            for _ in range(10):
+                s.residual(p, q, 0.5, 1.0)
+            ms = (time.perf_counter() - t0) / 10 * 1e3
+            rates["fp64_update_residual"] = {
+                "ms": round(ms, 4), "bytes_per_pt": 68,
+                "hbm_frac": round(68 * npts / ms / 1e6 / HBM_PEAK_GBS, 4),
+                "note": "host-timed, includes the partial-sum readback"}
+            ps.PSGridFree(p)
+            ps.PSGridFree(q)
+        s.free()
+    s = PoissonSolver(grids, 1, "fp32")
+    p32 = ps.PSGridNew(shape)
+    s.plain_sweeps(p32, 10)
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0)
+    s.plain_sweeps(p32, 100)
+    hip.record(e1)
+    ms = hip.ms(e0, e1) / 100
+    rates["plain_sweep_fp32"] = {
+        "ms": round(ms, 5), "Gpts_s": round(npts / ms / 1e6, 2),
+        "bytes_per_pt": 56,
+        "hbm_frac": round(56 * npts / ms / 1e6 / HBM_PEAK_GBS, 4)}
+    s.free()
+    for g in grids + [p32]:
+        ps.PSGridFree(g)
+    out["sweeps_512"] = rates
+    # time to 1e-6 at 128^3
+    n = 128
+    coefs = poisson_problem(n, n, n, nbatch=1, noise=0.05, gaussians=(4, 24),
+                            sigma=(2.0, 16.0), seed=5)
+    shape = (n, n, n)
+    grids = []
+    for c in coefs:
+        g = ps.PSGridNew(shape)
+        ps.PSGridCopyin(g, c)
+        grids.append(g)
+    solve = {}
+    log("solving a 128^3 problem to 1e-6: plain FP32, BF16, FP16")
+    s = PoissonSolver(grids, 1, "fp32")
+    p32 = ps.PSGridNew(shape)
+    st = s.solve_plain(p32, tol=1e-6, check=200)
+    solve["plain_fp32"] = {k: st[k] for k in ("converged", "stagnated",
+                                              "sweeps", "rel_residual",
+                                              "seconds")}
+    s.free()
+    ps.PSGridFree(p32)
+    for name in ("bf16", "fp16"):
+        s = PoissonSolver(grids, 1, name)
+        p64 = ps.PSGridNew(shape, np.float64)
+        st = s.solve_mixed(p64, tol=1e-6, inner=200)
+        solve[f"mixed_{name}"] = {k: st[k] for k in ("converged", "stagnated",
+                                                     "sweeps", "rel_residual",
+                                                     "seconds")}
+        s.free()
+        ps.PSGridFree(p64)
+    for g in grids:
+        ps.PSGridFree(g)
+    base = solve["plain_fp32"]["seconds"]
+    for k in ("mixed_bf16", "mixed_fp16"):
+        solve[k]["speedup_vs_plain_fp32"] = round(base / solve[k]["seconds"], 3)
+    out["time_to_1e-6_128"] = solve
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-poisson", action="store_true",
+                    help="skip the precision-tier leg")
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline leg")
     args = ap.parse_args()
@@ -271,6 +374,8 @@ def main() -> int:
                               HBM_PEAK_GBS, 4),
             "gosa_first_last": [float(gosa[0]), float(gosa[-1])]},
     }
+    if not args.no_poisson:
+        line["precision_tier"] = precision_tier(ps, hip)
     if not args.no_cpu:
         log("timing the CPU baselines")
         ref, omp = cpu_baselines(a, DIFFUSION_COEF)

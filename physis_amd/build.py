@@ -42,6 +42,8 @@ RUNTIME_SOURCES = [
 # Poisson solver extensions (precision tier, plain FP32 solve) over the
 # runtime's grids: libphysis_poisson_hip.
 POISSON_SOURCES = [os.path.join(CSRC, "poisson", "mixed_hip.hip")]
+# Learned warm start (U-Net forward/backward/Adam): libphysis_warmstart_hip.
+WARMSTART_SOURCES = [os.path.join(CSRC, "warmstart", "unet_hip.hip")]
 
 TRANSLATOR_SOURCES = [os.path.join(CSRC, "translator", f) for f in
                       ("lexer.cc", "parser.cc", "translator.cc", "physisc.cc")]
@@ -86,16 +88,25 @@ def build_runtime(objdir: str) -> str:
     return out
 
 
-def build_poisson(objdir: str) -> str:
+def _build_extension(sources: list[str], name: str, objdir: str) -> str:
+    """A shared library over libphysis_rt_hip from HIP sources."""
     objs = []
-    for src in POISSON_SOURCES:
+    for src in sources:
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
         _compile_object(src, obj)
         objs.append(obj)
-    out = os.path.join(LIB, "libphysis_poisson_hip.so")
+    out = os.path.join(LIB, f"lib{name}.so")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o",
           out, f"-L{LIB}", "-lphysis_rt_hip", "-Wl,-rpath,$ORIGIN"])
     return out
+
+
+def build_poisson(objdir: str) -> str:
+    return _build_extension(POISSON_SOURCES, "physis_poisson_hip", objdir)
+
+
+def build_warmstart(objdir: str) -> str:
+    return _build_extension(WARMSTART_SOURCES, "physis_warmstart_hip", objdir)
 
 
 def build_translator() -> str:
@@ -127,9 +138,10 @@ def build_all(verbose: bool = True) -> list[str]:
     os.makedirs(objdir, exist_ok=True)
     built = [build_translator(), build_runtime(objdir)]
     with ThreadPoolExecutor(max_workers=4) as ex:
-        poisson = ex.submit(build_poisson, objdir)
+        ext = [ex.submit(build_poisson, objdir),
+               ex.submit(build_warmstart, objdir)]
         built += list(ex.map(build_program, PROGRAM_SOURCES))
-        built.append(poisson.result())
+        built += [e.result() for e in ext]
     shutil.rmtree(objdir, ignore_errors=True)
     if verbose:
         for b in built:

@@ -1,0 +1,1045 @@
+// libphysis_warmstart_hip: the U-Net warm start of the Poisson solve
+// (include/physis/physis_warmstart.h), forward, backward and Adam on the GPU.
+//
+// Tensors are channels-last BF16 ([position][channel], position
+// ((n*D + z)*H + y)*W + x, channels a multiple of 8), so the 8 K-values an
+// MFMA lane feeds are one 16-byte load.
+//
+// 3x3x3 convolution as an implicit GEMM on v_mfma_f32_16x16x32_bf16:
+//   forward   D[16 x-positions][16 cout] += A[16 pos][32 k] B[32 k][16 cout],
+//             k = tap * CIN + ci, A gathered from the input with zero
+//             padding, B the weights stored [cout][k] (K padded to 32);
+//   backward  dX = the forward kernel on dZ with the flipped, transposed
+//             weights Wf[ci][tap' * COUT + co] = W[co][26 - tap'][ci];
+//   filter    dW[co][k] = sum over positions of dZ[p][co] X[p + off(k)][ci]:
+//             GEMM with the positions as K, split over blocks of 4096
+//             positions into FP32 partials summed in block order
+//             (deterministic).
+// MFMA lane layout (checked on the device against a host product): lane l
+// feeds A row l%16 and B column l%16 with k = 8*(l/16) .. +7, and receives
+// D rows 4*(l/16) .. +3 of column l%16.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "physis/hip/device.h"
+#include "physis/hip/stencil.h"
+#include "physis/physis_warmstart.h"
+#include "../runtime/runtime_hip.h"
+
+#pragma clang fp contract(off)
+
+namespace physis_unet {
+
+typedef unsigned short u16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kWgradChunk = 4096;  // positions per filter-gradient block
+constexpr int kPool = 4;           // pooling / upsampling factor per axis
+
+__device__ __forceinline__ float Bf2F(u16 h) {
+  return __uint_as_float((unsigned)h << 16);
+}
+__device__ __forceinline__ u16 F2Bf(float f) {
+  return physis_hip::Convert<physis_hip::bf16_t, float>(f).bits;
+}
+__device__ __forceinline__ bf16x8 Load8(const u16 *p) {
+  const uint4 v = *reinterpret_cast<const uint4 *>(p);
+  return __builtin_bit_cast(bf16x8, v);
+}
+__device__ __forceinline__ bf16x8 Zero8() {
+  return __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0));
+}
+
+constexpr int KPad(int cin) { return ((27 * cin + 31) / 32) * 32; }
+
+// ------------------------------------------------------------ convolution
+// One wave: 16 consecutive x of row (n, z, y) x all COUT.  Block: 4 waves on
+// 4 consecutive rows.  Grid: (ceil(W/16), ceil(H/4), N*D).
+template <int CIN, int COUT, bool RELU>
+__global__ __launch_bounds__(256) void ConvKernel(
+    const u16 *__restrict__ x, const u16 *__restrict__ wt,
+    const float *__restrict__ bias, u16 *__restrict__ y, int D, int H,
+    int W) {
+  constexpr int K = 27 * CIN, KB = (K + 31) / 32, KP = KB * 32;
+  constexpr int NT = (COUT + 15) / 16;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const int x0 = blockIdx.x * 16;
+  const int yy = blockIdx.y * 4 + wv;
+  const int nz = blockIdx.z;
+  const int n = nz / D, z = nz - n * D;
+  if (yy >= H) return;  // whole wave; the kernel has no barriers
+  f32x4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int xa = x0 + r;
+  for (int kb = 0; kb < KB; ++kb) {
+    const int k0 = kb * 32 + 8 * g;
+    bf16x8 a = Zero8();
+    if (k0 < K && xa < W) {
+      const int tap = k0 / CIN, ci0 = k0 - tap * CIN;
+      const int zs = z + tap / 9 - 1, ys = yy + (tap / 3) % 3 - 1,
+                xs = xa + tap % 3 - 1;
+      if (xs >= 0 && xs < W && ys >= 0 && ys < H && zs >= 0 && zs < D)
+        a = Load8(x + ((((long)n * D + zs) * H + ys) * W + xs) * CIN + ci0);
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int co = t * 16 + r;
+      const bf16x8 b =
+          co < COUT ? Load8(wt + (long)co * KP + kb * 32 + 8 * g) : Zero8();
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+    }
+  }
+  const long row = (((long)n * D + z) * H + yy) * W;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int co = t * 16 + r;
+    if (co >= COUT) continue;
+    const float bb = bias ? bias[co] : 0.0f;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int xo = x0 + 4 * g + v;
+      if (xo >= W) continue;
+      float val = acc[t][v] + bb;
+      if (RELU) val = val > 0.0f ? val : 0.0f;
+      y[(row + xo) * COUT + co] = F2Bf(val);
+    }
+  }
+}
+
+// Filter gradient partials: block b covers positions [b*chunk, (b+1)*chunk)
+// and writes part[b][co][k] (k < 27*CIN) and bpart[b][co].  Wave w takes
+// the column tiles w, w+4, ... of k.  A lane decodes its first position of
+// a 32-position step once and walks the next 7 incrementally.
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void WgradKernel(
+    const u16 *__restrict__ x, const u16 *__restrict__ dz,
+    float *__restrict__ part, float *__restrict__ bpart, int P, int D, int H,
+    int W, int chunk) {
+  constexpr int K = 27 * CIN, NTN = (K + 15) / 16, MT = (COUT + 15) / 16;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const int p0 = blockIdx.x * chunk;
+  const int p1 = p0 + chunk < P ? p0 + chunk : P;
+  for (int nt = wv; nt < NTN; nt += 4) {
+    f32x4 acc[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int col = nt * 16 + r;
+    const bool colv = col < K;
+    const int tap = colv ? col / CIN : 0, ci = colv ? col - tap * CIN : 0;
+    const int dzo = tap / 9 - 1, dyo = (tap / 3) % 3 - 1, dxo = tap % 3 - 1;
+    for (int kp = p0; kp < p1; kp += 32) {
+      bf16x8 b = Zero8();
+      bf16x8 a[MT];
+#pragma unroll
+      for (int m = 0; m < MT; ++m) a[m] = Zero8();
+      const int pb = kp + 8 * g;
+      int q = pb;
+      int xx = q % W;
+      q /= W;
+      int yy = q % H;
+      q /= H;
+      int zz = q % D;
+      int nn = q / D;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int pos = pb + j;
+        if (pos < p1) {
+          const int xs = xx + dxo, ys = yy + dyo, zs = zz + dzo;
+          if (colv && xs >= 0 && xs < W && ys >= 0 && ys < H && zs >= 0 &&
+              zs < D)
+            b[j] = __builtin_bit_cast(
+                __bf16, x[((long)((nn * D + zs) * H + ys) * W + xs) * CIN + ci]);
+#pragma unroll
+          for (int m = 0; m < MT; ++m) {
+            const int co = m * 16 + r;
+            if (co < COUT)
+              a[m][j] = __builtin_bit_cast(__bf16, dz[(long)pos * COUT + co]);
+          }
+        }
+        if (++xx == W) {
+          xx = 0;
+          if (++yy == H) {
+            yy = 0;
+            if (++zz == D) {
+              zz = 0;
+              ++nn;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+        acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b, acc[m], 0, 0, 0);
+    }
+    if (colv) {
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int co = m * 16 + 4 * g + v;
+          if (co < COUT)
+            part[((long)blockIdx.x * COUT + co) * K + col] = acc[m][v];
+        }
+    }
+  }
+  // bias: thread t sums channel t % COUT over every (256 / COUT)-th
+  // position, then the partial sums of each channel are added in order
+  constexpr int kLanes = 256 / COUT;
+  __shared__ float red[256];
+  float s = 0.0f;
+  const int c = threadIdx.x % COUT, lid = threadIdx.x / COUT;
+  if (lid < kLanes)
+    for (int p = p0 + lid; p < p1; p += kLanes)
+      s += Bf2F(dz[(long)p * COUT + c]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < COUT) {
+    float t = 0.0f;
+    for (int i = 0; i < kLanes; ++i) t += red[threadIdx.x + COUT * i];
+    bpart[(long)blockIdx.x * COUT + threadIdx.x] = t;
+  }
+}
+
+// out[i] = sum over blocks of part[b][i], in block order.
+__global__ void SumPartsKernel(const float *__restrict__ part, int nparts,
+                               long stride, float *__restrict__ out, long n) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.0f;
+  for (int b = 0; b < nparts; ++b) s += part[b * stride + i];
+  out[i] = s;
+}
+
+// ------------------------------------------------------ elementwise layers
+// 4x4x4 average pool: out[n][Z][Y][X][c], FP32 sum of the 64 inputs / 64.
+__global__ void PoolKernel(const u16 *__restrict__ in, u16 *__restrict__ out,
+                           int C, int Dc, int Hc, int Wc, long total) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int c = (int)(i % C);
+  long q = i / C;
+  const int X = (int)(q % Wc);
+  q /= Wc;
+  const int Y = (int)(q % Hc);
+  q /= Hc;
+  const int Z = (int)(q % Dc);
+  const long n = q / Dc;
+  const int D = Dc * kPool, H = Hc * kPool, W = Wc * kPool;
+  float s = 0.0f;
+  for (int dz = 0; dz < kPool; ++dz)
+    for (int dy = 0; dy < kPool; ++dy)
+      for (int dx = 0; dx < kPool; ++dx)
+        s += Bf2F(in[((((n * D + Z * kPool + dz) * H + Y * kPool + dy) * W) +
+                      X * kPool + dx) * C + c]);
+  out[i] = F2Bf(s * (1.0f / 64.0f));
+}
+
+// out[fine p][0:CA] = coarse[p / 4][0:CA]; out[p][CA:CA+CB] = skip[p].
+__global__ void UpConcatKernel(const u16 *__restrict__ coarse,
+                               const u16 *__restrict__ skip,
+                               u16 *__restrict__ out, int CA, int CB, int D,
+                               int H, int W, long total) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int C = CA + CB;
+  const int c = (int)(i % C);
+  const long p = i / C;
+  if (c >= CA) {
+    out[i] = skip[p * CB + (c - CA)];
+    return;
+  }
+  long q = p;
+  const int x = (int)(q % W);
+  q /= W;
+  const int y = (int)(q % H);
+  q /= H;
+  const int z = (int)(q % D);
+  const long n = q / D;
+  const int Dc = D / kPool, Hc = H / kPool, Wc = W / kPool;
+  out[i] = coarse[(((n * Dc + z / kPool) * Hc + y / kPool) * Wc + x / kPool) *
+                      CA + c];
+}
+
+// Gradient of the upsampled part of a concatenation: gc[coarse p][c] =
+// FP32 sum over the 64 fine positions of g[fine][c] (c < CA of C channels).
+__global__ void UpBackKernel(const u16 *__restrict__ g, u16 *__restrict__ gc,
+                             int CA, int C, int Dc, int Hc, int Wc,
+                             long total) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int c = (int)(i % CA);
+  long q = i / CA;
+  const int X = (int)(q % Wc);
+  q /= Wc;
+  const int Y = (int)(q % Hc);
+  q /= Hc;
+  const int Z = (int)(q % Dc);
+  const long n = q / Dc;
+  const int D = Dc * kPool, H = Hc * kPool, W = Wc * kPool;
+  float s = 0.0f;
+  for (int dz = 0; dz < kPool; ++dz)
+    for (int dy = 0; dy < kPool; ++dy)
+      for (int dx = 0; dx < kPool; ++dx)
+        s += Bf2F(g[((((n * D + Z * kPool + dz) * H + Y * kPool + dy) * W) +
+                     X * kPool + dx) * C + c]);
+  gc[i] = F2Bf(s);
+}
+
+// dz[p][c] = (gpool[p/4][c] / 64 + gskip[p][CA + c]) if act[p][c] > 0 else 0:
+// the gradient reaching a pooled encoder output through its pool and its
+// skip connection (gskip: the concatenation gradient, C channels), gated
+// by its ReLU.  FP32 sum, one rounding.
+__global__ void PoolBackSkipGateKernel(const u16 *__restrict__ gpool,
+                                       const u16 *__restrict__ gskip,
+                                       const u16 *__restrict__ act,
+                                       u16 *__restrict__ dz, int CB, int CA,
+                                       int D, int H, int W, long total) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int c = (int)(i % CB);
+  const long p = i / CB;
+  if (!(Bf2F(act[i]) > 0.0f)) {
+    dz[i] = 0;
+    return;
+  }
+  long q = p;
+  const int x = (int)(q % W);
+  q /= W;
+  const int y = (int)(q % H);
+  q /= H;
+  const int z = (int)(q % D);
+  const long n = q / D;
+  const int Dc = D / kPool, Hc = H / kPool, Wc = W / kPool;
+  const float gp = Bf2F(gpool[(((n * Dc + z / kPool) * Hc + y / kPool) * Wc +
+                               x / kPool) * CB + c]);
+  const float gs = Bf2F(gskip[p * (CA + CB) + CA + c]);
+  dz[i] = F2Bf(gp * (1.0f / 64.0f) + gs);
+}
+
+// g[i] *= (act[i] > 0): ReLU gradient, in place.
+__global__ void GateKernel(u16 *__restrict__ g, const u16 *__restrict__ act,
+                           long n) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n && !(Bf2F(act[i]) > 0.0f)) g[i] = 0;
+}
+
+// Network input: ch0 = rhs, ch1 = bnd, ch2..7 = 0 (problems [first, ...)).
+__global__ void InputKernel(const float *__restrict__ rhs,
+                            const float *__restrict__ bnd,
+                            u16 *__restrict__ x, long off, long P) {
+  const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  uint4 v = make_uint4(0, 0, 0, 0);
+  v.x = (unsigned)F2Bf(rhs[off + p]) | ((unsigned)F2Bf(bnd[off + p]) << 16);
+  *reinterpret_cast<uint4 *>(x + p * 8) = v;
+}
+
+// 1x1 convolution 8 -> 1: out[p] = sum_c w[c] in[p][c] + b (FP32).
+__global__ void HeadKernel(const u16 *__restrict__ in,
+                           const float *__restrict__ w,
+                           float *__restrict__ out, long P) {
+  const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  float s = 0.0f;
+  for (int c = 0; c < 8; ++c) s += w[c] * Bf2F(in[p * 8 + c]);
+  out[p] = s + w[8];
+}
+
+// Loss partials and dout = 2 (out - t) / P with t = target * inv_scale.
+__global__ __launch_bounds__(256) void LossKernel(
+    const float *__restrict__ out, const float *__restrict__ target,
+    long toff, float inv_scale, float *__restrict__ dout,
+    double *__restrict__ part, long P) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < P;
+       p += (long)gridDim.x * blockDim.x) {
+    const float d = out[p] - target[toff + p] * inv_scale;
+    dout[p] = 2.0f * d / (float)P;
+    s += (double)d * d;
+  }
+  s = physis_hip::BlockReduce<double, PS_SUM, 256>(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// Head backward: din[p][c] = w[c] dout[p] (BF16, gated by act), and
+// per-block partials of dw[c] = sum dout[p] in[p][c], db = sum dout[p].
+__global__ __launch_bounds__(256) void HeadBackKernel(
+    const u16 *__restrict__ in, const float *__restrict__ w,
+    const float *__restrict__ dout, u16 *__restrict__ din,
+    float *__restrict__ part, long P) {
+  __shared__ float red[9][4];
+  float acc[9];
+  for (int c = 0; c < 9; ++c) acc[c] = 0.0f;
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < P;
+       p += (long)gridDim.x * blockDim.x) {
+    const float d = dout[p];
+    for (int c = 0; c < 8; ++c) {
+      const float a = Bf2F(in[p * 8 + c]);
+      acc[c] += d * a;
+      din[p * 8 + c] = a > 0.0f ? F2Bf(w[c] * d) : (u16)0;
+    }
+    acc[8] += d;
+  }
+  for (int c = 0; c < 9; ++c) {
+    const float s = physis_hip::BlockReduce<float, PS_SUM, 256>(acc[c], red[c]);
+    if (threadIdx.x == 0) part[blockIdx.x * 9 + c] = s;
+  }
+}
+
+// Prediction: p[off + i] = scale * out[i] * bnd[off + i] (FP32 or FP64 p).
+template <typename T>
+__global__ void PredictKernel(const float *__restrict__ out,
+                              const float *__restrict__ bnd, T *__restrict__ p,
+                              long off, float scale, long P) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < P) p[off + i] = (T)(scale * out[i]) * (T)bnd[off + i];
+}
+
+// Weights: wt[co][k] = bf16(W[co][k]) (zero for k >= 27 cin) and the
+// flipped, transposed wf[ci][tap' cout + co] = bf16(W[co][26 - tap'][ci]).
+__global__ void PackKernel(const float *__restrict__ w, u16 *__restrict__ wt,
+                           u16 *__restrict__ wf, int cin, int cout) {
+  const int K = 27 * cin, kp = KPad(cin), kf = KPad(cout);
+  const int n1 = cout * kp, n2 = cin * kf;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n1 + n2;
+       i += gridDim.x * blockDim.x) {
+    if (i < n1) {
+      const int co = i / kp, k = i - co * kp;
+      wt[i] = k < K ? F2Bf(w[co * K + k]) : (u16)0;
+    } else {
+      const int j = i - n1, ci = j / kf, k = j - ci * kf;
+      u16 v = 0;
+      if (k < 27 * cout) {
+        const int tp = k / cout, co = k - tp * cout;
+        v = F2Bf(w[co * K + (26 - tp) * cin + ci]);
+      }
+      wf[j] = v;
+    }
+  }
+}
+
+__global__ void AdamKernel(float *__restrict__ p, const float *__restrict__ g,
+                           float *__restrict__ m, float *__restrict__ v,
+                           long n, float lr, float c1, float c2) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = 0.9f * m[i] + 0.1f * gi;
+  const float vi = 0.999f * v[i] + 0.001f * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+}
+
+// ------------------------------------------------------------ dispatch
+hipStream_t Stream() { return (hipStream_t)__PSHipStream(); }
+void Check() { PS_HIP_CHECK(hipGetLastError()); }
+unsigned Blocks(long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+template <int CIN, int COUT>
+void ConvT(const u16 *x, const u16 *wt, const float *b, bool relu, u16 *y,
+           int N, int D, int H, int W) {
+  const dim3 grid((W + 15) / 16, (H + 3) / 4, N * D), block(256);
+  if (relu)
+    hipLaunchKernelGGL((ConvKernel<CIN, COUT, true>), grid, block, 0, Stream(),
+                       x, wt, b, y, D, H, W);
+  else
+    hipLaunchKernelGGL((ConvKernel<CIN, COUT, false>), grid, block, 0,
+                       Stream(), x, wt, b, y, D, H, W);
+  Check();
+}
+
+template <int CIN, int COUT>
+void WgradT(const u16 *x, const u16 *dz, float *part, float *bpart, int N,
+            int D, int H, int W, int *nparts) {
+  const long P = (long)N * D * H * W;
+  if (P >= (1L << 31) - kWgradChunk)
+    PS_FATAL("filter gradient: %ld positions exceed 32-bit indexing", P);
+  const int nb = (int)((P + kWgradChunk - 1) / kWgradChunk);
+  hipLaunchKernelGGL((WgradKernel<CIN, COUT>), dim3(nb), dim3(256), 0,
+                     Stream(), x, dz, part, bpart, (int)P, D, H, W,
+                     kWgradChunk);
+  Check();
+  *nparts = nb;
+}
+
+#define PS_UNET_PAIRS(X) \
+  X(8, 8) X(8, 16) X(16, 16) X(16, 32) X(32, 32) X(48, 16) X(24, 8) \
+  X(16, 8) X(32, 16) X(16, 48) X(8, 24)
+
+void Conv(int cin, int cout, const u16 *x, const u16 *wt, const float *b,
+          bool relu, u16 *y, int N, int D, int H, int W) {
+#define PS_CASE(I, O) \
+  if (cin == I && cout == O) return ConvT<I, O>(x, wt, b, relu, y, N, D, H, W);
+  PS_UNET_PAIRS(PS_CASE)
+#undef PS_CASE
+  PS_FATAL("no convolution kernel for %d -> %d channels", cin, cout);
+}
+
+void Wgrad(int cin, int cout, const u16 *x, const u16 *dz, float *part,
+           float *bpart, int N, int D, int H, int W, int *nparts) {
+#define PS_CASE(I, O) \
+  if (cin == I && cout == O) \
+    return WgradT<I, O>(x, dz, part, bpart, N, D, H, W, nparts);
+  PS_UNET_PAIRS(PS_CASE)
+#undef PS_CASE
+  PS_FATAL("no filter-gradient kernel for %d -> %d channels", cin, cout);
+}
+
+struct Layer {
+  int cin, cout;
+  size_t w_off, b_off;  // into the flat parameter vector
+  u16 *wt = nullptr, *wf = nullptr;
+};
+
+void *Alloc(size_t bytes) {
+  void *p = nullptr;
+  if (bytes == 0) bytes = 16;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return p;
+}
+
+uint64_t SplitMix(uint64_t &s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+}  // namespace physis_unet
+
+using namespace physis_unet;
+
+struct __PSUNet {
+  int nx, ny, nzs, max_batch;
+  float out_scale;
+  std::vector<Layer> layers;  // 10 convolutions
+  size_t head_off, nparams;
+  float *params = nullptr, *grad = nullptr, *adam_m = nullptr,
+        *adam_v = nullptr;
+  long step = 0;
+  // activations (BF16) at batch max_batch
+  u16 *x0 = nullptr, *a[10] = {}, *q1 = nullptr, *q2 = nullptr,
+      *c2 = nullptr, *c1 = nullptr;
+  float *out = nullptr, *dout = nullptr;
+  // gradient scratch
+  u16 *g1a = nullptr, *g1b = nullptr, *gc1 = nullptr;  // level 1
+  u16 *g2a = nullptr, *g2b = nullptr, *gc2 = nullptr;  // level 2
+  u16 *g3a = nullptr, *g3b = nullptr;                  // level 3
+  float *part = nullptr, *bpart = nullptr;
+  double *lpart = nullptr, *lhost = nullptr;
+  size_t part_elems = 0;
+};
+
+namespace {
+
+const int kLayerCh[10][2] = {{8, 8},  {8, 8},   {8, 16},  {16, 16}, {16, 32},
+                             {32, 32}, {48, 16}, {16, 16}, {24, 8},  {8, 8}};
+constexpr int kLossBlocks = 1024;
+constexpr int kHeadBlocks = 1024;
+
+void Upload(__PSUNet *net) {  // master weights -> BF16 copies
+  for (auto &L : net->layers) {
+    hipLaunchKernelGGL(PackKernel, dim3(64), dim3(256), 0, Stream(),
+                       net->params + L.w_off, L.wt, L.wf, L.cin, L.cout);
+    Check();
+  }
+}
+
+struct Dims {
+  int N, D1, H1, W1, D2, H2, W2, D3, H3, W3;
+  long P1, P2, P3;
+};
+
+Dims DimsOf(const __PSUNet *net, int count) {
+  Dims d;
+  d.N = count;
+  d.D1 = net->nzs; d.H1 = net->ny; d.W1 = net->nx;
+  d.D2 = d.D1 / kPool; d.H2 = d.H1 / kPool; d.W2 = d.W1 / kPool;
+  d.D3 = d.D2 / kPool; d.H3 = d.H2 / kPool; d.W3 = d.W2 / kPool;
+  d.P1 = (long)count * d.D1 * d.H1 * d.W1;
+  d.P2 = (long)count * d.D2 * d.H2 * d.W2;
+  d.P3 = (long)count * d.D3 * d.H3 * d.W3;
+  return d;
+}
+
+void ConvL(const __PSUNet *net, int l, const u16 *in, u16 *out, int N, int D,
+           int H, int W) {
+  const Layer &L = net->layers[l];
+  Conv(L.cin, L.cout, in, L.wt, net->params + L.b_off, true, out, N, D, H, W);
+}
+
+void CheckGrid(const __PSUNet *net, const __PSGrid *g, int first, int count,
+               const char *what) {
+  if (!g || !g->p || g->num_dims != 3 || g->dim[0] != net->nx ||
+      g->dim[1] != net->ny || g->dim[2] % net->nzs != 0)
+    PS_FATAL("U-Net: %s is not a 3-D grid of %d x %d x (k * %d)", what,
+             net->nx, net->ny, net->nzs);
+  const int nb = g->dim[2] / net->nzs;
+  if (first < 0 || count < 1 || count > net->max_batch || first + count > nb)
+    PS_FATAL("U-Net: problems [%d, %d) outside 0..%d or above the batch %d",
+             first, first + count, nb, net->max_batch);
+}
+
+// Forward pass of problems [first, first+count): fills the activations and
+// net->out.
+void Forward(__PSUNet *net, const __PSGrid *rhs, const __PSGrid *bnd,
+             int first, int count) {
+  CheckGrid(net, rhs, first, count, "rhs");
+  CheckGrid(net, bnd, first, count, "bnd");
+  if (rhs->type != PS_FLOAT || bnd->type != PS_FLOAT)
+    PS_FATAL("U-Net: rhs and bnd must be float grids");
+  const Dims d = DimsOf(net, count);
+  const long off = (long)first * net->nx * net->ny * net->nzs;
+  hipLaunchKernelGGL(InputKernel, dim3(Blocks(d.P1)), dim3(256), 0, Stream(),
+                     (const float *)rhs->p, (const float *)bnd->p, net->x0,
+                     off, d.P1);
+  Check();
+  ConvL(net, 0, net->x0, net->a[0], d.N, d.D1, d.H1, d.W1);
+  ConvL(net, 1, net->a[0], net->a[1], d.N, d.D1, d.H1, d.W1);
+  hipLaunchKernelGGL(PoolKernel, dim3(Blocks(d.P2 * 8)), dim3(256), 0,
+                     Stream(), net->a[1], net->q1, 8, d.D2, d.H2, d.W2,
+                     d.P2 * 8);
+  Check();
+  ConvL(net, 2, net->q1, net->a[2], d.N, d.D2, d.H2, d.W2);
+  ConvL(net, 3, net->a[2], net->a[3], d.N, d.D2, d.H2, d.W2);
+  hipLaunchKernelGGL(PoolKernel, dim3(Blocks(d.P3 * 16)), dim3(256), 0,
+                     Stream(), net->a[3], net->q2, 16, d.D3, d.H3, d.W3,
+                     d.P3 * 16);
+  Check();
+  ConvL(net, 4, net->q2, net->a[4], d.N, d.D3, d.H3, d.W3);
+  ConvL(net, 5, net->a[4], net->a[5], d.N, d.D3, d.H3, d.W3);
+  hipLaunchKernelGGL(UpConcatKernel, dim3(Blocks(d.P2 * 48)), dim3(256), 0,
+                     Stream(), net->a[5], net->a[3], net->c2, 32, 16, d.D2,
+                     d.H2, d.W2, d.P2 * 48);
+  Check();
+  ConvL(net, 6, net->c2, net->a[6], d.N, d.D2, d.H2, d.W2);
+  ConvL(net, 7, net->a[6], net->a[7], d.N, d.D2, d.H2, d.W2);
+  hipLaunchKernelGGL(UpConcatKernel, dim3(Blocks(d.P1 * 24)), dim3(256), 0,
+                     Stream(), net->a[7], net->a[1], net->c1, 16, 8, d.D1,
+                     d.H1, d.W1, d.P1 * 24);
+  Check();
+  ConvL(net, 8, net->c1, net->a[8], d.N, d.D1, d.H1, d.W1);
+  ConvL(net, 9, net->a[8], net->a[9], d.N, d.D1, d.H1, d.W1);
+  hipLaunchKernelGGL(HeadKernel, dim3(Blocks(d.P1)), dim3(256), 0, Stream(),
+                     net->a[9], net->params + net->head_off, net->out, d.P1);
+  Check();
+}
+
+// Filter gradient of layer l from input `in` and gated output gradient dz.
+void LayerWgrad(__PSUNet *net, int l, const u16 *in, const u16 *dz, int N,
+                int D, int H, int W) {
+  const Layer &L = net->layers[l];
+  int np = 0;
+  Wgrad(L.cin, L.cout, in, dz, net->part, net->bpart, N, D, H, W, &np);
+  const long nw = (long)L.cout * 27 * L.cin;
+  hipLaunchKernelGGL(SumPartsKernel, dim3(Blocks(nw)), dim3(256), 0, Stream(),
+                     net->part, np, nw, net->grad + L.w_off, nw);
+  Check();
+  hipLaunchKernelGGL(SumPartsKernel, dim3(1), dim3(64), 0, Stream(),
+                     net->bpart, np, (long)L.cout, net->grad + L.b_off,
+                     (long)L.cout);
+  Check();
+}
+
+// Data gradient of layer l: dx = conv(dz, flipped W), no bias, no ReLU.
+void LayerDgrad(__PSUNet *net, int l, const u16 *dz, u16 *dx, int N, int D,
+                int H, int W) {
+  const Layer &L = net->layers[l];
+  Conv(L.cout, L.cin, dz, L.wf, nullptr, false, dx, N, D, H, W);
+}
+
+void Gate(u16 *g, const u16 *act, long n) {
+  hipLaunchKernelGGL(GateKernel, dim3(Blocks(n)), dim3(256), 0, Stream(), g,
+                     act, n);
+  Check();
+}
+
+// Loss of the current forward pass against target, and the full gradient
+// into net->grad.
+double Backward(__PSUNet *net, const __PSGrid *target, int first, int count) {
+  CheckGrid(net, target, first, count, "target");
+  if (target->type != PS_FLOAT) PS_FATAL("U-Net: target must be a float grid");
+  const Dims d = DimsOf(net, count);
+  const long off = (long)first * net->nx * net->ny * net->nzs;
+  hipLaunchKernelGGL(LossKernel, dim3(kLossBlocks), dim3(256), 0, Stream(),
+                     net->out, (const float *)target->p, off,
+                     1.0f / net->out_scale, net->dout, net->lpart, d.P1);
+  Check();
+  // head: g1a = gated dD1 (layer 9's dz), head partials in part
+  hipLaunchKernelGGL(HeadBackKernel, dim3(kHeadBlocks), dim3(256), 0,
+                     Stream(), net->a[9], net->params + net->head_off,
+                     net->dout, net->g1a, net->part, d.P1);
+  Check();
+  hipLaunchKernelGGL(SumPartsKernel, dim3(1), dim3(64), 0, Stream(),
+                     net->part, kHeadBlocks, 9L, net->grad + net->head_off,
+                     9L);
+  Check();
+  // level 1 decoder
+  LayerWgrad(net, 9, net->a[8], net->g1a, d.N, d.D1, d.H1, d.W1);
+  LayerDgrad(net, 9, net->g1a, net->g1b, d.N, d.D1, d.H1, d.W1);
+  Gate(net->g1b, net->a[8], d.P1 * 8);
+  LayerWgrad(net, 8, net->c1, net->g1b, d.N, d.D1, d.H1, d.W1);
+  LayerDgrad(net, 8, net->g1b, net->gc1, d.N, d.D1, d.H1, d.W1);  // 24 ch
+  hipLaunchKernelGGL(UpBackKernel, dim3(Blocks(d.P2 * 16)), dim3(256), 0,
+                     Stream(), net->gc1, net->g2a, 16, 24, d.D2, d.H2, d.W2,
+                     d.P2 * 16);
+  Check();
+  // level 2 decoder
+  Gate(net->g2a, net->a[7], d.P2 * 16);
+  LayerWgrad(net, 7, net->a[6], net->g2a, d.N, d.D2, d.H2, d.W2);
+  LayerDgrad(net, 7, net->g2a, net->g2b, d.N, d.D2, d.H2, d.W2);
+  Gate(net->g2b, net->a[6], d.P2 * 16);
+  LayerWgrad(net, 6, net->c2, net->g2b, d.N, d.D2, d.H2, d.W2);
+  LayerDgrad(net, 6, net->g2b, net->gc2, d.N, d.D2, d.H2, d.W2);  // 48 ch
+  hipLaunchKernelGGL(UpBackKernel, dim3(Blocks(d.P3 * 32)), dim3(256), 0,
+                     Stream(), net->gc2, net->g3a, 32, 48, d.D3, d.H3, d.W3,
+                     d.P3 * 32);
+  Check();
+  // level 3
+  Gate(net->g3a, net->a[5], d.P3 * 32);
+  LayerWgrad(net, 5, net->a[4], net->g3a, d.N, d.D3, d.H3, d.W3);
+  LayerDgrad(net, 5, net->g3a, net->g3b, d.N, d.D3, d.H3, d.W3);
+  Gate(net->g3b, net->a[4], d.P3 * 32);
+  LayerWgrad(net, 4, net->q2, net->g3b, d.N, d.D3, d.H3, d.W3);
+  LayerDgrad(net, 4, net->g3b, net->g3a, d.N, d.D3, d.H3, d.W3);  // 16 ch
+  // level 2 encoder: E2 receives the pool gradient and the skip gradient
+  hipLaunchKernelGGL(PoolBackSkipGateKernel, dim3(Blocks(d.P2 * 16)),
+                     dim3(256), 0, Stream(), net->g3a, net->gc2, net->a[3],
+                     net->g2a, 16, 32, d.D2, d.H2, d.W2, d.P2 * 16);
+  Check();
+  LayerWgrad(net, 3, net->a[2], net->g2a, d.N, d.D2, d.H2, d.W2);
+  LayerDgrad(net, 3, net->g2a, net->g2b, d.N, d.D2, d.H2, d.W2);
+  Gate(net->g2b, net->a[2], d.P2 * 16);
+  LayerWgrad(net, 2, net->q1, net->g2b, d.N, d.D2, d.H2, d.W2);
+  LayerDgrad(net, 2, net->g2b, net->g2a, d.N, d.D2, d.H2, d.W2);  // 8 ch
+  // level 1 encoder
+  hipLaunchKernelGGL(PoolBackSkipGateKernel, dim3(Blocks(d.P1 * 8)),
+                     dim3(256), 0, Stream(), net->g2a, net->gc1, net->a[1],
+                     net->g1a, 8, 16, d.D1, d.H1, d.W1, d.P1 * 8);
+  Check();
+  LayerWgrad(net, 1, net->a[0], net->g1a, d.N, d.D1, d.H1, d.W1);
+  LayerDgrad(net, 1, net->g1a, net->g1b, d.N, d.D1, d.H1, d.W1);
+  Gate(net->g1b, net->a[0], d.P1 * 8);
+  LayerWgrad(net, 0, net->x0, net->g1b, d.N, d.D1, d.H1, d.W1);
+  PS_HIP_CHECK(hipMemcpyAsync(net->lhost, net->lpart,
+                              sizeof(double) * kLossBlocks,
+                              hipMemcpyDeviceToHost, Stream()));
+  PS_HIP_CHECK(hipStreamSynchronize(Stream()));
+  double loss = 0.0;
+  for (int b = 0; b < kLossBlocks; ++b) loss += net->lhost[b];
+  return loss / (double)d.P1;
+}
+
+void FreeNet(__PSUNet *net) {
+  (void)hipStreamSynchronize(Stream());
+  std::vector<void *> ptrs = {net->params, net->grad, net->adam_m,
+                              net->adam_v, net->x0, net->q1, net->q2,
+                              net->c2, net->c1, net->out, net->dout, net->g1a,
+                              net->g1b, net->gc1, net->g2a, net->g2b,
+                              net->gc2, net->g3a, net->g3b, net->part,
+                              net->bpart, net->lpart};
+  for (int i = 0; i < 10; ++i) ptrs.push_back(net->a[i]);
+  for (auto &L : net->layers) {
+    ptrs.push_back(L.wt);
+    ptrs.push_back(L.wf);
+  }
+  for (void *p : ptrs)
+    if (p) PS_HIP_CHECK(hipFree(p));
+  if (net->lhost) PS_HIP_CHECK(hipHostFree(net->lhost));
+  delete net;
+}
+
+// Host-side device buffer for the single-layer entry points.
+struct DevBuf {
+  void *p = nullptr;
+  explicit DevBuf(size_t bytes) {
+    p = Alloc(bytes);
+    if (!p) PS_FATAL("out of device memory");
+  }
+  ~DevBuf() { (void)hipFree(p); }
+};
+
+void CheckLayerArgs(int n, int d, int h, int w, int cin, int cout) {
+  if (n < 1 || d < 1 || h < 1 || w < 1 || cin % 8 || cout < 1)
+    PS_FATAL("convolution: bad sizes n=%d d=%d h=%d w=%d cin=%d cout=%d", n,
+             d, h, w, cin, cout);
+}
+
+}  // namespace
+
+extern "C" {
+
+__PSUNet *PSUNetNew(int nx, int ny, int nzs, int max_batch, float out_scale,
+                    uint64_t seed) {
+  if (nx % 16 || ny % 16 || nzs % 16 || nx < 16 || ny < 16 || nzs < 16 ||
+      max_batch < 1 || !(out_scale > 0.0f)) {
+    std::fprintf(stderr,
+                 "[physis-hip] PSUNetNew: sizes must be positive multiples "
+                 "of 16, batch >= 1, scale > 0\n");
+    return nullptr;
+  }
+  __PSUNet *net = new __PSUNet();
+  net->nx = nx;
+  net->ny = ny;
+  net->nzs = nzs;
+  net->max_batch = max_batch;
+  net->out_scale = out_scale;
+  size_t off = 0;
+  for (int l = 0; l < 10; ++l) {
+    Layer L;
+    L.cin = kLayerCh[l][0];
+    L.cout = kLayerCh[l][1];
+    L.w_off = off;
+    off += (size_t)L.cout * 27 * L.cin;
+    L.b_off = off;
+    off += L.cout;
+    net->layers.push_back(L);
+  }
+  net->head_off = off;
+  off += 9;
+  net->nparams = off;
+  // He-normal weights (std sqrt(2 / fan_in)), zero biases; head std 1/sqrt(8)
+  std::vector<float> h(off, 0.0f);
+  uint64_t s = seed;
+  auto normal = [&]() {
+    const double u1 = ((SplitMix(s) >> 11) + 1.0) * (1.0 / 9007199254740993.0);
+    const double u2 = (SplitMix(s) >> 11) * (1.0 / 9007199254740992.0);
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+  };
+  for (auto &L : net->layers) {
+    const double sd = std::sqrt(2.0 / (27.0 * L.cin));
+    for (size_t i = 0; i < (size_t)L.cout * 27 * L.cin; ++i)
+      h[L.w_off + i] = (float)(sd * normal());
+  }
+  for (int c = 0; c < 8; ++c) h[net->head_off + c] = (float)(normal() / std::sqrt(8.0));
+  const Dims d = DimsOf(net, max_batch);
+  bool ok = true;
+  auto A16 = [&](long elems) {
+    u16 *p = (u16 *)Alloc(sizeof(u16) * (size_t)elems);
+    ok = ok && p;
+    return p;
+  };
+  auto A32 = [&](long elems) {
+    float *p = (float *)Alloc(sizeof(float) * (size_t)elems);
+    ok = ok && p;
+    return p;
+  };
+  net->params = A32(off);
+  net->grad = A32(off);
+  net->adam_m = A32(off);
+  net->adam_v = A32(off);
+  for (auto &L : net->layers) {
+    L.wt = A16((long)L.cout * KPad(L.cin));
+    L.wf = A16((long)L.cin * KPad(L.cout));
+  }
+  net->x0 = A16(d.P1 * 8);
+  const long act[10] = {d.P1 * 8,  d.P1 * 8,  d.P2 * 16, d.P2 * 16, d.P3 * 32,
+                        d.P3 * 32, d.P2 * 16, d.P2 * 16, d.P1 * 8,  d.P1 * 8};
+  for (int l = 0; l < 10; ++l) net->a[l] = A16(act[l]);
+  net->q1 = A16(d.P2 * 8);
+  net->q2 = A16(d.P3 * 16);
+  net->c2 = A16(d.P2 * 48);
+  net->c1 = A16(d.P1 * 24);
+  net->out = A32(d.P1);
+  net->dout = A32(d.P1);
+  net->g1a = A16(d.P1 * 8);
+  net->g1b = A16(d.P1 * 8);
+  net->gc1 = A16(d.P1 * 24);
+  net->g2a = A16(d.P2 * 16);
+  net->g2b = A16(d.P2 * 16);
+  net->gc2 = A16(d.P2 * 48);
+  net->g3a = A16(d.P3 * 32);
+  net->g3b = A16(d.P3 * 32);
+  size_t pe = (size_t)kHeadBlocks * 9;
+  for (int l = 0; l < 10; ++l) {
+    const long P = l < 2 || l > 7 ? d.P1 : (l < 4 || l > 5 ? d.P2 : d.P3);
+    const size_t nb = (size_t)((P + kWgradChunk - 1) / kWgradChunk);
+    pe = std::max(pe, nb * net->layers[l].cout * 27 * net->layers[l].cin);
+  }
+  net->part_elems = pe;
+  net->part = A32((long)pe);
+  net->bpart = A32((long)((d.P1 + kWgradChunk - 1) / kWgradChunk) * 32);
+  net->lpart = (double *)Alloc(sizeof(double) * kLossBlocks);
+  ok = ok && net->lpart;
+  if (ok && hipHostMalloc((void **)&net->lhost, sizeof(double) * kLossBlocks) !=
+                hipSuccess) {
+    (void)hipGetLastError();
+    ok = false;
+  }
+  if (!ok) {
+    std::fprintf(stderr, "[physis-hip] PSUNetNew: out of device memory\n");
+    FreeNet(net);
+    return nullptr;
+  }
+  PS_HIP_CHECK(hipMemcpy(net->params, h.data(), sizeof(float) * off,
+                         hipMemcpyHostToDevice));
+  PS_HIP_CHECK(hipMemset(net->adam_m, 0, sizeof(float) * off));
+  PS_HIP_CHECK(hipMemset(net->adam_v, 0, sizeof(float) * off));
+  PS_HIP_CHECK(hipMemset(net->grad, 0, sizeof(float) * off));
+  Upload(net);
+  PS_HIP_CHECK(hipStreamSynchronize(Stream()));
+  return net;
+}
+
+void PSUNetFree(__PSUNet *net) {
+  if (net) FreeNet(net);
+}
+
+size_t PSUNetNumParams(const __PSUNet *net) { return net->nparams; }
+
+void PSUNetGetParams(const __PSUNet *net, float *host) {
+  PS_HIP_CHECK(hipStreamSynchronize(Stream()));
+  PS_HIP_CHECK(hipMemcpy(host, net->params, sizeof(float) * net->nparams,
+                         hipMemcpyDeviceToHost));
+}
+
+void PSUNetSetParams(__PSUNet *net, const float *host) {
+  PS_HIP_CHECK(hipMemcpy(net->params, host, sizeof(float) * net->nparams,
+                         hipMemcpyHostToDevice));
+  Upload(net);
+  PS_HIP_CHECK(hipStreamSynchronize(Stream()));
+}
+
+void PSUNetPredict(__PSUNet *net, const __PSGrid *rhs, const __PSGrid *bnd,
+                   int first, int count, __PSGrid *p) {
+  CheckGrid(net, p, first, count, "p");
+  if (p->type != PS_FLOAT && p->type != PS_DOUBLE)
+    PS_FATAL("U-Net: p must be a float or double grid");
+  Forward(net, rhs, bnd, first, count);
+  const Dims d = DimsOf(net, count);
+  const long off = (long)first * net->nx * net->ny * net->nzs;
+  if (p->type == PS_DOUBLE)
+    hipLaunchKernelGGL(PredictKernel<double>, dim3(Blocks(d.P1)), dim3(256), 0,
+                       Stream(), net->out, (const float *)bnd->p,
+                       (double *)p->p, off, net->out_scale, d.P1);
+  else
+    hipLaunchKernelGGL(PredictKernel<float>, dim3(Blocks(d.P1)), dim3(256), 0,
+                       Stream(), net->out, (const float *)bnd->p,
+                       (float *)p->p, off, net->out_scale, d.P1);
+  Check();
+  PS_HIP_CHECK(hipStreamSynchronize(Stream()));
+}
+
+double PSUNetLossGrad(__PSUNet *net, const __PSGrid *rhs, const __PSGrid *bnd,
+                      const __PSGrid *target, int first, int count,
+                      float *grad_host) {
+  Forward(net, rhs, bnd, first, count);
+  const double loss = Backward(net, target, first, count);
+  if (grad_host)
+    PS_HIP_CHECK(hipMemcpy(grad_host, net->grad, sizeof(float) * net->nparams,
+                           hipMemcpyDeviceToHost));
+  return loss;
+}
+
+double PSUNetTrainStep(__PSUNet *net, const __PSGrid *rhs,
+                       const __PSGrid *bnd, const __PSGrid *target, int first,
+                       int count, float lr) {
+  Forward(net, rhs, bnd, first, count);
+  const double loss = Backward(net, target, first, count);
+  net->step += 1;
+  const float c1 = 1.0f - std::pow(0.9f, (float)net->step);
+  const float c2 = 1.0f - std::pow(0.999f, (float)net->step);
+  hipLaunchKernelGGL(AdamKernel, dim3(Blocks((long)net->nparams)), dim3(256),
+                     0, Stream(), net->params, net->grad, net->adam_m,
+                     net->adam_v, (long)net->nparams, lr, c1, c2);
+  Check();
+  Upload(net);
+  PS_HIP_CHECK(hipStreamSynchronize(Stream()));
+  return loss;
+}
+
+void PSConv3dForwardHost(const uint16_t *x, const uint16_t *w,
+                         const float *bias, int relu, int n, int d, int h,
+                         int wd, int cin, int cout, uint16_t *y) {
+  CheckLayerArgs(n, d, h, wd, cin, cout);
+  const long P = (long)n * d * h * wd;
+  const size_t nw = (size_t)cout * 27 * cin;
+  DevBuf dx(P * cin * 2), dw32(nw * 4), dwt((size_t)cout * KPad(cin) * 2),
+      dwf((size_t)cin * KPad(cout) * 2), db(cout * 4), dy(P * cout * 2);
+  std::vector<float> wf(nw);
+  for (size_t i = 0; i < nw; ++i) {
+    const uint32_t u = (uint32_t)w[i] << 16;
+    std::memcpy(&wf[i], &u, 4);
+  }
+  PS_HIP_CHECK(hipMemcpy(dx.p, x, P * cin * 2, hipMemcpyHostToDevice));
+  PS_HIP_CHECK(hipMemcpy(dw32.p, wf.data(), nw * 4, hipMemcpyHostToDevice));
+  PS_HIP_CHECK(hipMemcpy(db.p, bias, cout * 4, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(PackKernel, dim3(64), dim3(256), 0, Stream(),
+                     (const float *)dw32.p, (u16 *)dwt.p, (u16 *)dwf.p, cin,
+                     cout);
+  Check();
+  Conv(cin, cout, (const u16 *)dx.p, (const u16 *)dwt.p, (const float *)db.p,
+       relu != 0, (u16 *)dy.p, n, d, h, wd);
+  PS_HIP_CHECK(hipStreamSynchronize(Stream()));
+  PS_HIP_CHECK(hipMemcpy(y, dy.p, P * cout * 2, hipMemcpyDeviceToHost));
+}
+
+void PSConv3dBackwardDataHost(const uint16_t *dy, const uint16_t *w, int n,
+                              int d, int h, int wd, int cin, int cout,
+                              uint16_t *dx) {
+  CheckLayerArgs(n, d, h, wd, cin, cout);
+  if (cout % 8) PS_FATAL("convolution: cout must be a multiple of 8");
+  const long P = (long)n * d * h * wd;
+  const size_t nw = (size_t)cout * 27 * cin;
+  DevBuf ddy(P * cout * 2), dw32(nw * 4), dwt((size_t)cout * KPad(cin) * 2),
+      dwf((size_t)cin * KPad(cout) * 2), ddx(P * cin * 2);
+  std::vector<float> wf(nw);
+  for (size_t i = 0; i < nw; ++i) {
+    const uint32_t u = (uint32_t)w[i] << 16;
+    std::memcpy(&wf[i], &u, 4);
+  }
+  PS_HIP_CHECK(hipMemcpy(ddy.p, dy, P * cout * 2, hipMemcpyHostToDevice));
+  PS_HIP_CHECK(hipMemcpy(dw32.p, wf.data(), nw * 4, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(PackKernel, dim3(64), dim3(256), 0, Stream(),
+                     (const float *)dw32.p, (u16 *)dwt.p, (u16 *)dwf.p, cin,
+                     cout);
+  Check();
+  Conv(cout, cin, (const u16 *)ddy.p, (const u16 *)dwf.p, nullptr, false,
+       (u16 *)ddx.p, n, d, h, wd);
+  PS_HIP_CHECK(hipStreamSynchronize(Stream()));
+  PS_HIP_CHECK(hipMemcpy(dx, ddx.p, P * cin * 2, hipMemcpyDeviceToHost));
+}
+
+void PSConv3dBackwardFilterHost(const uint16_t *x, const uint16_t *dy, int n,
+                                int d, int h, int wd, int cin, int cout,
+                                float *dw, float *db) {
+  CheckLayerArgs(n, d, h, wd, cin, cout);
+  const long P = (long)n * d * h * wd;
+  const long nb = (P + kWgradChunk - 1) / kWgradChunk;
+  const size_t nw = (size_t)cout * 27 * cin;
+  DevBuf ddx(P * cin * 2), ddy(P * cout * 2), part(nb * nw * 4),
+      bpart(nb * cout * 4), dw_d(nw * 4), db_d(cout * 4);
+  PS_HIP_CHECK(hipMemcpy(ddx.p, x, P * cin * 2, hipMemcpyHostToDevice));
+  PS_HIP_CHECK(hipMemcpy(ddy.p, dy, P * cout * 2, hipMemcpyHostToDevice));
+  int np = 0;
+  Wgrad(cin, cout, (const u16 *)ddx.p, (const u16 *)ddy.p, (float *)part.p,
+        (float *)bpart.p, n, d, h, wd, &np);
+  hipLaunchKernelGGL(SumPartsKernel, dim3(Blocks((long)nw)), dim3(256), 0,
+                     Stream(), (const float *)part.p, np, (long)nw,
+                     (float *)dw_d.p, (long)nw);
+  Check();
+  hipLaunchKernelGGL(SumPartsKernel, dim3(1), dim3(64), 0, Stream(),
+                     (const float *)bpart.p, np, (long)cout, (float *)db_d.p,
+                     (long)cout);
+  Check();
+  PS_HIP_CHECK(hipStreamSynchronize(Stream()));
+  PS_HIP_CHECK(hipMemcpy(dw, dw_d.p, nw * 4, hipMemcpyDeviceToHost));
+  PS_HIP_CHECK(hipMemcpy(db, db_d.p, cout * 4, hipMemcpyDeviceToHost));
+}
+
+}  // extern "C"

@@ -57,3 +57,15 @@ def test_poisson_library_exports_its_header():
     lib = C.CDLL(os.path.join(LIB, "libphysis_poisson_hip.so"))
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_warmstart_library_exports_its_header():
+    text = open(os.path.join(ROOT, "include", "physis",
+                             "physis_warmstart.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = sorted(set(re.findall(r"\b(PS(?:UNet|Conv3d)\w+)\s*\(", text)))
+    assert "PSUNetTrainStep" in names and "PSConv3dForwardHost" in names
+    C.CDLL(os.path.join(LIB, "libphysis_rt_hip.so"), mode=C.RTLD_GLOBAL)
+    lib = C.CDLL(os.path.join(LIB, "libphysis_warmstart_hip.so"))
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
