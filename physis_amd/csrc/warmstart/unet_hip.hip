@@ -40,7 +40,7 @@ typedef unsigned short u16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kWgradChunk = 4096;  // positions per filter-gradient block
+constexpr int kWgradChunk = 16384;  // max positions per filter-gradient block
 constexpr int kPool = 4;           // pooling / upsampling factor per axis
 
 __device__ __forceinline__ float Bf2F(u16 h) {
@@ -60,8 +60,10 @@ __device__ __forceinline__ bf16x8 Zero8() {
 constexpr int KPad(int cin) { return ((27 * cin + 31) / 32) * 32; }
 
 // ------------------------------------------------------------ convolution
-// One wave: 16 consecutive x of row (n, z, y) x all COUT.  Block: 4 waves on
-// 4 consecutive rows.  Grid: (ceil(W/16), ceil(H/4), N*D).
+// One wave: row (n, z, y), in segments of 16 consecutive x, all COUT.  Block:
+// 4 waves on 4 consecutive rows.  Grid: (ceil(H/4), N*D).  When the weight
+// fragments of all K steps fit (KB*NT <= 24 registers of 8 BF16), they are
+// loaded once per wave and reused across the row's segments.
 template <int CIN, int COUT, bool RELU>
 __global__ __launch_bounds__(256) void ConvKernel(
     const u16 *__restrict__ x, const u16 *__restrict__ wt,
@@ -69,56 +71,81 @@ __global__ __launch_bounds__(256) void ConvKernel(
     int W) {
   constexpr int K = 27 * CIN, KB = (K + 31) / 32, KP = KB * 32;
   constexpr int NT = (COUT + 15) / 16;
+  constexpr bool kPre = KB * NT <= 24;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int g = lane >> 4, r = lane & 15;
-  const int x0 = blockIdx.x * 16;
-  const int yy = blockIdx.y * 4 + wv;
-  const int nz = blockIdx.z;
+  const int yy = blockIdx.x * 4 + wv;
+  const int nz = blockIdx.y;
   const int n = nz / D, z = nz - n * D;
   if (yy >= H) return;  // whole wave; the kernel has no barriers
-  f32x4 acc[NT];
+  bf16x8 bw[kPre ? KB : 1][NT];
+  if constexpr (kPre) {
 #pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const int xa = x0 + r;
-  for (int kb = 0; kb < KB; ++kb) {
-    const int k0 = kb * 32 + 8 * g;
-    bf16x8 a = Zero8();
-    if (k0 < K && xa < W) {
-      const int tap = k0 / CIN, ci0 = k0 - tap * CIN;
-      const int zs = z + tap / 9 - 1, ys = yy + (tap / 3) % 3 - 1,
-                xs = xa + tap % 3 - 1;
-      if (xs >= 0 && xs < W && ys >= 0 && ys < H && zs >= 0 && zs < D)
-        a = Load8(x + ((((long)n * D + zs) * H + ys) * W + xs) * CIN + ci0);
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int co = t * 16 + r;
+        bw[kb][t] = co < COUT ? Load8(wt + (long)co * KP + kb * 32 + 8 * g)
+                              : Zero8();
+      }
+  }
+  float bb[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int co = t * 16 + r;
+    bb[t] = (bias && co < COUT) ? bias[co] : 0.0f;
+  }
+  const long row = (((long)n * D + z) * H + yy) * W;
+  for (int x0 = 0; x0 < W; x0 += 16) {
+    f32x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int xa = x0 + r;
+#pragma unroll 4
+    for (int kb = 0; kb < KB; ++kb) {
+      const int k0 = kb * 32 + 8 * g;
+      bf16x8 a = Zero8();
+      if (k0 < K && xa < W) {
+        const int tap = k0 / CIN, ci0 = k0 - tap * CIN;
+        const int zs = z + tap / 9 - 1, ys = yy + (tap / 3) % 3 - 1,
+                  xs = xa + tap % 3 - 1;
+        if (xs >= 0 && xs < W && ys >= 0 && ys < H && zs >= 0 && zs < D)
+          a = Load8(x + ((((long)n * D + zs) * H + ys) * W + xs) * CIN + ci0);
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        bf16x8 b;
+        if constexpr (kPre) {
+          b = bw[kb][t];
+        } else {
+          const int co = t * 16 + r;
+          b = co < COUT ? Load8(wt + (long)co * KP + kb * 32 + 8 * g)
+                        : Zero8();
+        }
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+      }
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       const int co = t * 16 + r;
-      const bf16x8 b =
-          co < COUT ? Load8(wt + (long)co * KP + kb * 32 + 8 * g) : Zero8();
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
-    }
-  }
-  const long row = (((long)n * D + z) * H + yy) * W;
+      if (co >= COUT) continue;
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int co = t * 16 + r;
-    if (co >= COUT) continue;
-    const float bb = bias ? bias[co] : 0.0f;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int xo = x0 + 4 * g + v;
-      if (xo >= W) continue;
-      float val = acc[t][v] + bb;
-      if (RELU) val = val > 0.0f ? val : 0.0f;
-      y[(row + xo) * COUT + co] = F2Bf(val);
+      for (int v = 0; v < 4; ++v) {
+        const int xo = x0 + 4 * g + v;
+        if (xo >= W) continue;
+        float val = acc[t][v] + bb[t];
+        if (RELU) val = val > 0.0f ? val : 0.0f;
+        y[(row + xo) * COUT + co] = F2Bf(val);
+      }
     }
   }
 }
 
-// Filter gradient partials: block b covers positions [b*chunk, (b+1)*chunk)
-// and writes part[b][co][k] (k < 27*CIN) and bpart[b][co].  Wave w takes
-// the column tiles w, w+4, ... of k.  A lane decodes its first position of
-// a 32-position step once and walks the next 7 incrementally.
+// Filter gradient partials: block (b, t) covers positions [b*chunk,
+// (b+1)*chunk) and column tiles 4t .. 4t+3 of k (one per wave) and writes
+// part[b][co][k]; blocks (b, 0) also write bpart[b][co].  A lane decodes its
+// first position of a 32-position step once and walks the next 7
+// incrementally.
 template <int CIN, int COUT>
 __global__ __launch_bounds__(256) void WgradKernel(
     const u16 *__restrict__ x, const u16 *__restrict__ dz,
@@ -129,7 +156,8 @@ __global__ __launch_bounds__(256) void WgradKernel(
   const int g = lane >> 4, r = lane & 15;
   const int p0 = blockIdx.x * chunk;
   const int p1 = p0 + chunk < P ? p0 + chunk : P;
-  for (int nt = wv; nt < NTN; nt += 4) {
+  const int nt = blockIdx.y * 4 + wv;
+  if (nt < NTN) {
     f32x4 acc[MT];
 #pragma unroll
     for (int m = 0; m < MT; ++m) acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -192,8 +220,10 @@ __global__ __launch_bounds__(256) void WgradKernel(
         }
     }
   }
-  // bias: thread t sums channel t % COUT over every (256 / COUT)-th
-  // position, then the partial sums of each channel are added in order
+  // bias (blocks of the first tile group): thread t sums channel t % COUT
+  // over every (256 / COUT)-th position, then the partial sums of each
+  // channel are added in order
+  if (blockIdx.y != 0) return;
   constexpr int kLanes = 256 / COUT;
   __shared__ float red[256];
   float s = 0.0f;
@@ -210,7 +240,20 @@ __global__ __launch_bounds__(256) void WgradKernel(
   }
 }
 
-// out[i] = sum over blocks of part[b][i], in block order.
+// Deterministic sum over blocks of part[b][i]: stage 1 sums segment s of
+// the blocks in order into tmp[s][i], stage 2 sums the segments in order.
+__global__ void SumSegmentsKernel(const float *__restrict__ part, int nparts,
+                                  long stride, int seglen,
+                                  float *__restrict__ tmp, long n) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b0 = blockIdx.y * seglen;
+  const int b1 = b0 + seglen < nparts ? b0 + seglen : nparts;
+  float s = 0.0f;
+  for (int b = b0; b < b1; ++b) s += part[b * stride + i];
+  tmp[blockIdx.y * n + i] = s;
+}
+
 __global__ void SumPartsKernel(const float *__restrict__ part, int nparts,
                                long stride, float *__restrict__ out, long n) {
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -221,116 +264,161 @@ __global__ void SumPartsKernel(const float *__restrict__ part, int nparts,
 }
 
 // ------------------------------------------------------ elementwise layers
-// 4x4x4 average pool: out[n][Z][Y][X][c], FP32 sum of the 64 inputs / 64.
+// Threads own 8 consecutive channels (one 16-byte vector) of one position;
+// positions are decoded with 32-bit arithmetic (< 2^31 positions).
+struct V8 {
+  float v[8];
+};
+__device__ __forceinline__ V8 Load8F(const u16 *p) {
+  const uint4 u = *reinterpret_cast<const uint4 *>(p);
+  const unsigned w[4] = {u.x, u.y, u.z, u.w};
+  V8 r;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    r.v[2 * i] = __uint_as_float(w[i] << 16);
+    r.v[2 * i + 1] = __uint_as_float(w[i] & 0xffff0000u);
+  }
+  return r;
+}
+__device__ __forceinline__ void Store8F(u16 *p, const V8 &r) {
+  unsigned w[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    w[i] = (unsigned)F2Bf(r.v[2 * i]) | ((unsigned)F2Bf(r.v[2 * i + 1]) << 16);
+  *reinterpret_cast<uint4 *>(p) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+struct Pos {
+  int n, z, y, x;
+};
+__device__ __forceinline__ Pos Decode(int p, int D, int H, int W) {
+  Pos r;
+  r.x = p % W;
+  p /= W;
+  r.y = p % H;
+  p /= H;
+  r.z = p % D;
+  r.n = p / D;
+  return r;
+}
+
+// 4x4x4 average pool: out[coarse][c] = (FP32 sum of the 64 inputs) / 64.
 __global__ void PoolKernel(const u16 *__restrict__ in, u16 *__restrict__ out,
-                           int C, int Dc, int Hc, int Wc, long total) {
-  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+                           int C, int Dc, int Hc, int Wc, int total) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int G = C / 8;
   if (i >= total) return;
-  const int c = (int)(i % C);
-  long q = i / C;
-  const int X = (int)(q % Wc);
-  q /= Wc;
-  const int Y = (int)(q % Hc);
-  q /= Hc;
-  const int Z = (int)(q % Dc);
-  const long n = q / Dc;
+  const int gi = i % G;
+  const Pos q = Decode(i / G, Dc, Hc, Wc);
   const int D = Dc * kPool, H = Hc * kPool, W = Wc * kPool;
-  float s = 0.0f;
+  V8 s;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s.v[k] = 0.0f;
   for (int dz = 0; dz < kPool; ++dz)
     for (int dy = 0; dy < kPool; ++dy)
-      for (int dx = 0; dx < kPool; ++dx)
-        s += Bf2F(in[((((n * D + Z * kPool + dz) * H + Y * kPool + dy) * W) +
-                      X * kPool + dx) * C + c]);
-  out[i] = F2Bf(s * (1.0f / 64.0f));
+      for (int dx = 0; dx < kPool; ++dx) {
+        const long f = (((long)q.n * D + q.z * kPool + dz) * H + q.y * kPool +
+                        dy) * W + q.x * kPool + dx;
+        const V8 a = Load8F(in + f * C + gi * 8);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s.v[k] += a.v[k];
+      }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s.v[k] *= (1.0f / 64.0f);
+  Store8F(out + (long)i * 8, s);
 }
 
 // out[fine p][0:CA] = coarse[p / 4][0:CA]; out[p][CA:CA+CB] = skip[p].
 __global__ void UpConcatKernel(const u16 *__restrict__ coarse,
                                const u16 *__restrict__ skip,
                                u16 *__restrict__ out, int CA, int CB, int D,
-                               int H, int W, long total) {
-  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+                               int H, int W, int total) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int G = (CA + CB) / 8, GA = CA / 8;
   if (i >= total) return;
-  const int C = CA + CB;
-  const int c = (int)(i % C);
-  const long p = i / C;
-  if (c >= CA) {
-    out[i] = skip[p * CB + (c - CA)];
-    return;
+  const int gi = i % G, p = i / G;
+  uint4 v;
+  if (gi >= GA) {
+    v = *reinterpret_cast<const uint4 *>(skip + (long)p * CB + (gi - GA) * 8);
+  } else {
+    const Pos q = Decode(p, D, H, W);
+    const int Dc = D / kPool, Hc = H / kPool, Wc = W / kPool;
+    const long c = (((long)q.n * Dc + q.z / kPool) * Hc + q.y / kPool) * Wc +
+                   q.x / kPool;
+    v = *reinterpret_cast<const uint4 *>(coarse + c * CA + gi * 8);
   }
-  long q = p;
-  const int x = (int)(q % W);
-  q /= W;
-  const int y = (int)(q % H);
-  q /= H;
-  const int z = (int)(q % D);
-  const long n = q / D;
-  const int Dc = D / kPool, Hc = H / kPool, Wc = W / kPool;
-  out[i] = coarse[(((n * Dc + z / kPool) * Hc + y / kPool) * Wc + x / kPool) *
-                      CA + c];
+  *reinterpret_cast<uint4 *>(out + (long)i * 8) = v;
 }
 
-// Gradient of the upsampled part of a concatenation: gc[coarse p][c] =
-// FP32 sum over the 64 fine positions of g[fine][c] (c < CA of C channels).
+// Gradient of the upsampled part of a concatenation: gc[coarse][c] = FP32
+// sum over the 64 fine positions of g[fine][c], c < CA of C channels.
 __global__ void UpBackKernel(const u16 *__restrict__ g, u16 *__restrict__ gc,
                              int CA, int C, int Dc, int Hc, int Wc,
-                             long total) {
-  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+                             int total) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int GA = CA / 8;
   if (i >= total) return;
-  const int c = (int)(i % CA);
-  long q = i / CA;
-  const int X = (int)(q % Wc);
-  q /= Wc;
-  const int Y = (int)(q % Hc);
-  q /= Hc;
-  const int Z = (int)(q % Dc);
-  const long n = q / Dc;
+  const int gi = i % GA;
+  const Pos q = Decode(i / GA, Dc, Hc, Wc);
   const int D = Dc * kPool, H = Hc * kPool, W = Wc * kPool;
-  float s = 0.0f;
+  V8 s;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s.v[k] = 0.0f;
   for (int dz = 0; dz < kPool; ++dz)
     for (int dy = 0; dy < kPool; ++dy)
-      for (int dx = 0; dx < kPool; ++dx)
-        s += Bf2F(g[((((n * D + Z * kPool + dz) * H + Y * kPool + dy) * W) +
-                     X * kPool + dx) * C + c]);
-  gc[i] = F2Bf(s);
+      for (int dx = 0; dx < kPool; ++dx) {
+        const long f = (((long)q.n * D + q.z * kPool + dz) * H + q.y * kPool +
+                        dy) * W + q.x * kPool + dx;
+        const V8 a = Load8F(g + f * C + gi * 8);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s.v[k] += a.v[k];
+      }
+  Store8F(gc + (long)i * 8, s);
 }
 
 // dz[p][c] = (gpool[p/4][c] / 64 + gskip[p][CA + c]) if act[p][c] > 0 else 0:
 // the gradient reaching a pooled encoder output through its pool and its
-// skip connection (gskip: the concatenation gradient, C channels), gated
-// by its ReLU.  FP32 sum, one rounding.
+// skip connection (gskip: the concatenation gradient, CA + CB channels),
+// gated by its ReLU.  FP32 sum, one rounding.
 __global__ void PoolBackSkipGateKernel(const u16 *__restrict__ gpool,
                                        const u16 *__restrict__ gskip,
                                        const u16 *__restrict__ act,
                                        u16 *__restrict__ dz, int CB, int CA,
-                                       int D, int H, int W, long total) {
-  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+                                       int D, int H, int W, int total) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int G = CB / 8;
   if (i >= total) return;
-  const int c = (int)(i % CB);
-  const long p = i / CB;
-  if (!(Bf2F(act[i]) > 0.0f)) {
-    dz[i] = 0;
-    return;
-  }
-  long q = p;
-  const int x = (int)(q % W);
-  q /= W;
-  const int y = (int)(q % H);
-  q /= H;
-  const int z = (int)(q % D);
-  const long n = q / D;
+  const int gi = i % G, p = i / G;
+  const Pos q = Decode(p, D, H, W);
   const int Dc = D / kPool, Hc = H / kPool, Wc = W / kPool;
-  const float gp = Bf2F(gpool[(((n * Dc + z / kPool) * Hc + y / kPool) * Wc +
-                               x / kPool) * CB + c]);
-  const float gs = Bf2F(gskip[p * (CA + CB) + CA + c]);
-  dz[i] = F2Bf(gp * (1.0f / 64.0f) + gs);
+  const long c = (((long)q.n * Dc + q.z / kPool) * Hc + q.y / kPool) * Wc +
+                 q.x / kPool;
+  const V8 gp = Load8F(gpool + c * CB + gi * 8);
+  const V8 gs = Load8F(gskip + (long)p * (CA + CB) + CA + gi * 8);
+  const V8 a = Load8F(act + (long)i * 8);
+  V8 o;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    o.v[k] = a.v[k] > 0.0f ? gp.v[k] * (1.0f / 64.0f) + gs.v[k] : 0.0f;
+  Store8F(dz + (long)i * 8, o);
 }
 
-// g[i] *= (act[i] > 0): ReLU gradient, in place.
+// g *= (act > 0): ReLU gradient, in place, 8 values per thread.
 __global__ void GateKernel(u16 *__restrict__ g, const u16 *__restrict__ act,
-                           long n) {
+                           long n8) {
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i < n && !(Bf2F(act[i]) > 0.0f)) g[i] = 0;
+  if (i >= n8) return;
+  uint4 gv = *reinterpret_cast<const uint4 *>(g + i * 8);
+  const uint4 av = *reinterpret_cast<const uint4 *>(act + i * 8);
+  unsigned *gw = reinterpret_cast<unsigned *>(&gv);
+  const unsigned aw[4] = {av.x, av.y, av.z, av.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const bool lo = __uint_as_float(aw[k] << 16) > 0.0f;
+    const bool hi = __uint_as_float(aw[k] & 0xffff0000u) > 0.0f;
+    gw[k] = (lo ? gw[k] & 0xffffu : 0u) | (hi ? gw[k] & 0xffff0000u : 0u);
+  }
+  *reinterpret_cast<uint4 *>(g + i * 8) = gv;
 }
 
 // Network input: ch0 = rhs, ch1 = bnd, ch2..7 = 0 (problems [first, ...)).
@@ -450,7 +538,7 @@ unsigned Blocks(long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 template <int CIN, int COUT>
 void ConvT(const u16 *x, const u16 *wt, const float *b, bool relu, u16 *y,
            int N, int D, int H, int W) {
-  const dim3 grid((W + 15) / 16, (H + 3) / 4, N * D), block(256);
+  const dim3 grid((H + 3) / 4, N * D), block(256);
   if (relu)
     hipLaunchKernelGGL((ConvKernel<CIN, COUT, true>), grid, block, 0, Stream(),
                        x, wt, b, y, D, H, W);
@@ -460,18 +548,45 @@ void ConvT(const u16 *x, const u16 *wt, const float *b, bool relu, u16 *y,
   Check();
 }
 
+// Positions per filter-gradient block: about P / 512 (a multiple of 32),
+// between 256 and kWgradChunk, so that small levels still spread over many
+// blocks.
+int WgradChunk(long P) {
+  long c = ((P / 512 + 31) / 32) * 32;
+  if (c < 256) c = 256;
+  if (c > kWgradChunk) c = kWgradChunk;
+  return (int)c;
+}
+
 template <int CIN, int COUT>
 void WgradT(const u16 *x, const u16 *dz, float *part, float *bpart, int N,
             int D, int H, int W, int *nparts) {
+  constexpr int NTN = (27 * CIN + 15) / 16;
   const long P = (long)N * D * H * W;
   if (P >= (1L << 31) - kWgradChunk)
     PS_FATAL("filter gradient: %ld positions exceed 32-bit indexing", P);
-  const int nb = (int)((P + kWgradChunk - 1) / kWgradChunk);
-  hipLaunchKernelGGL((WgradKernel<CIN, COUT>), dim3(nb), dim3(256), 0,
-                     Stream(), x, dz, part, bpart, (int)P, D, H, W,
-                     kWgradChunk);
+  const int chunk = WgradChunk(P);
+  const int nb = (int)((P + chunk - 1) / chunk);
+  hipLaunchKernelGGL((WgradKernel<CIN, COUT>), dim3(nb, (NTN + 3) / 4),
+                     dim3(256), 0, Stream(), x, dz, part, bpart, (int)P, D, H,
+                     W, chunk);
   Check();
   *nparts = nb;
+}
+
+// out[0..n) = sum over nparts blocks of part[b * n + i], via tmp
+// (kSumSegments * n floats).
+constexpr int kSumSegments = 64;
+void SumParts(const float *part, int nparts, long n, float *tmp, float *out) {
+  const int seg = nparts < kSumSegments ? nparts : kSumSegments;
+  const int seglen = (nparts + seg - 1) / seg;
+  const int nseg = (nparts + seglen - 1) / seglen;
+  hipLaunchKernelGGL(SumSegmentsKernel, dim3(Blocks(n), nseg), dim3(256), 0,
+                     Stream(), part, nparts, n, seglen, tmp, n);
+  Check();
+  hipLaunchKernelGGL(SumPartsKernel, dim3(Blocks(n)), dim3(256), 0, Stream(),
+                     tmp, nseg, n, out, n);
+  Check();
 }
 
 #define PS_UNET_PAIRS(X) \
@@ -540,7 +655,7 @@ struct __PSUNet {
   u16 *g1a = nullptr, *g1b = nullptr, *gc1 = nullptr;  // level 1
   u16 *g2a = nullptr, *g2b = nullptr, *gc2 = nullptr;  // level 2
   u16 *g3a = nullptr, *g3b = nullptr;                  // level 3
-  float *part = nullptr, *bpart = nullptr;
+  float *part = nullptr, *bpart = nullptr, *tmp = nullptr;
   double *lpart = nullptr, *lhost = nullptr;
   size_t part_elems = 0;
 };
@@ -611,27 +726,26 @@ void Forward(__PSUNet *net, const __PSGrid *rhs, const __PSGrid *bnd,
   Check();
   ConvL(net, 0, net->x0, net->a[0], d.N, d.D1, d.H1, d.W1);
   ConvL(net, 1, net->a[0], net->a[1], d.N, d.D1, d.H1, d.W1);
-  hipLaunchKernelGGL(PoolKernel, dim3(Blocks(d.P2 * 8)), dim3(256), 0,
-                     Stream(), net->a[1], net->q1, 8, d.D2, d.H2, d.W2,
-                     d.P2 * 8);
+  hipLaunchKernelGGL(PoolKernel, dim3(Blocks(d.P2)), dim3(256), 0, Stream(),
+                     net->a[1], net->q1, 8, d.D2, d.H2, d.W2, (int)d.P2);
   Check();
   ConvL(net, 2, net->q1, net->a[2], d.N, d.D2, d.H2, d.W2);
   ConvL(net, 3, net->a[2], net->a[3], d.N, d.D2, d.H2, d.W2);
-  hipLaunchKernelGGL(PoolKernel, dim3(Blocks(d.P3 * 16)), dim3(256), 0,
+  hipLaunchKernelGGL(PoolKernel, dim3(Blocks(d.P3 * 2)), dim3(256), 0,
                      Stream(), net->a[3], net->q2, 16, d.D3, d.H3, d.W3,
-                     d.P3 * 16);
+                     (int)(d.P3 * 2));
   Check();
   ConvL(net, 4, net->q2, net->a[4], d.N, d.D3, d.H3, d.W3);
   ConvL(net, 5, net->a[4], net->a[5], d.N, d.D3, d.H3, d.W3);
-  hipLaunchKernelGGL(UpConcatKernel, dim3(Blocks(d.P2 * 48)), dim3(256), 0,
+  hipLaunchKernelGGL(UpConcatKernel, dim3(Blocks(d.P2 * 6)), dim3(256), 0,
                      Stream(), net->a[5], net->a[3], net->c2, 32, 16, d.D2,
-                     d.H2, d.W2, d.P2 * 48);
+                     d.H2, d.W2, (int)(d.P2 * 6));
   Check();
   ConvL(net, 6, net->c2, net->a[6], d.N, d.D2, d.H2, d.W2);
   ConvL(net, 7, net->a[6], net->a[7], d.N, d.D2, d.H2, d.W2);
-  hipLaunchKernelGGL(UpConcatKernel, dim3(Blocks(d.P1 * 24)), dim3(256), 0,
+  hipLaunchKernelGGL(UpConcatKernel, dim3(Blocks(d.P1 * 3)), dim3(256), 0,
                      Stream(), net->a[7], net->a[1], net->c1, 16, 8, d.D1,
-                     d.H1, d.W1, d.P1 * 24);
+                     d.H1, d.W1, (int)(d.P1 * 3));
   Check();
   ConvL(net, 8, net->c1, net->a[8], d.N, d.D1, d.H1, d.W1);
   ConvL(net, 9, net->a[8], net->a[9], d.N, d.D1, d.H1, d.W1);
@@ -647,13 +761,8 @@ void LayerWgrad(__PSUNet *net, int l, const u16 *in, const u16 *dz, int N,
   int np = 0;
   Wgrad(L.cin, L.cout, in, dz, net->part, net->bpart, N, D, H, W, &np);
   const long nw = (long)L.cout * 27 * L.cin;
-  hipLaunchKernelGGL(SumPartsKernel, dim3(Blocks(nw)), dim3(256), 0, Stream(),
-                     net->part, np, nw, net->grad + L.w_off, nw);
-  Check();
-  hipLaunchKernelGGL(SumPartsKernel, dim3(1), dim3(64), 0, Stream(),
-                     net->bpart, np, (long)L.cout, net->grad + L.b_off,
-                     (long)L.cout);
-  Check();
+  SumParts(net->part, np, nw, net->tmp, net->grad + L.w_off);
+  SumParts(net->bpart, np, (long)L.cout, net->tmp, net->grad + L.b_off);
 }
 
 // Data gradient of layer l: dx = conv(dz, flipped W), no bias, no ReLU.
@@ -663,9 +772,9 @@ void LayerDgrad(__PSUNet *net, int l, const u16 *dz, u16 *dx, int N, int D,
   Conv(L.cout, L.cin, dz, L.wf, nullptr, false, dx, N, D, H, W);
 }
 
-void Gate(u16 *g, const u16 *act, long n) {
-  hipLaunchKernelGGL(GateKernel, dim3(Blocks(n)), dim3(256), 0, Stream(), g,
-                     act, n);
+void Gate(u16 *g, const u16 *act, long n) {  // n: a multiple of 8
+  hipLaunchKernelGGL(GateKernel, dim3(Blocks(n / 8)), dim3(256), 0, Stream(),
+                     g, act, n / 8);
   Check();
 }
 
@@ -685,19 +794,16 @@ double Backward(__PSUNet *net, const __PSGrid *target, int first, int count) {
                      Stream(), net->a[9], net->params + net->head_off,
                      net->dout, net->g1a, net->part, d.P1);
   Check();
-  hipLaunchKernelGGL(SumPartsKernel, dim3(1), dim3(64), 0, Stream(),
-                     net->part, kHeadBlocks, 9L, net->grad + net->head_off,
-                     9L);
-  Check();
+  SumParts(net->part, kHeadBlocks, 9L, net->tmp, net->grad + net->head_off);
   // level 1 decoder
   LayerWgrad(net, 9, net->a[8], net->g1a, d.N, d.D1, d.H1, d.W1);
   LayerDgrad(net, 9, net->g1a, net->g1b, d.N, d.D1, d.H1, d.W1);
   Gate(net->g1b, net->a[8], d.P1 * 8);
   LayerWgrad(net, 8, net->c1, net->g1b, d.N, d.D1, d.H1, d.W1);
   LayerDgrad(net, 8, net->g1b, net->gc1, d.N, d.D1, d.H1, d.W1);  // 24 ch
-  hipLaunchKernelGGL(UpBackKernel, dim3(Blocks(d.P2 * 16)), dim3(256), 0,
+  hipLaunchKernelGGL(UpBackKernel, dim3(Blocks(d.P2 * 2)), dim3(256), 0,
                      Stream(), net->gc1, net->g2a, 16, 24, d.D2, d.H2, d.W2,
-                     d.P2 * 16);
+                     (int)(d.P2 * 2));
   Check();
   // level 2 decoder
   Gate(net->g2a, net->a[7], d.P2 * 16);
@@ -706,9 +812,9 @@ double Backward(__PSUNet *net, const __PSGrid *target, int first, int count) {
   Gate(net->g2b, net->a[6], d.P2 * 16);
   LayerWgrad(net, 6, net->c2, net->g2b, d.N, d.D2, d.H2, d.W2);
   LayerDgrad(net, 6, net->g2b, net->gc2, d.N, d.D2, d.H2, d.W2);  // 48 ch
-  hipLaunchKernelGGL(UpBackKernel, dim3(Blocks(d.P3 * 32)), dim3(256), 0,
+  hipLaunchKernelGGL(UpBackKernel, dim3(Blocks(d.P3 * 4)), dim3(256), 0,
                      Stream(), net->gc2, net->g3a, 32, 48, d.D3, d.H3, d.W3,
-                     d.P3 * 32);
+                     (int)(d.P3 * 4));
   Check();
   // level 3
   Gate(net->g3a, net->a[5], d.P3 * 32);
@@ -718,9 +824,9 @@ double Backward(__PSUNet *net, const __PSGrid *target, int first, int count) {
   LayerWgrad(net, 4, net->q2, net->g3b, d.N, d.D3, d.H3, d.W3);
   LayerDgrad(net, 4, net->g3b, net->g3a, d.N, d.D3, d.H3, d.W3);  // 16 ch
   // level 2 encoder: E2 receives the pool gradient and the skip gradient
-  hipLaunchKernelGGL(PoolBackSkipGateKernel, dim3(Blocks(d.P2 * 16)),
+  hipLaunchKernelGGL(PoolBackSkipGateKernel, dim3(Blocks(d.P2 * 2)),
                      dim3(256), 0, Stream(), net->g3a, net->gc2, net->a[3],
-                     net->g2a, 16, 32, d.D2, d.H2, d.W2, d.P2 * 16);
+                     net->g2a, 16, 32, d.D2, d.H2, d.W2, (int)(d.P2 * 2));
   Check();
   LayerWgrad(net, 3, net->a[2], net->g2a, d.N, d.D2, d.H2, d.W2);
   LayerDgrad(net, 3, net->g2a, net->g2b, d.N, d.D2, d.H2, d.W2);
@@ -728,9 +834,9 @@ double Backward(__PSUNet *net, const __PSGrid *target, int first, int count) {
   LayerWgrad(net, 2, net->q1, net->g2b, d.N, d.D2, d.H2, d.W2);
   LayerDgrad(net, 2, net->g2b, net->g2a, d.N, d.D2, d.H2, d.W2);  // 8 ch
   // level 1 encoder
-  hipLaunchKernelGGL(PoolBackSkipGateKernel, dim3(Blocks(d.P1 * 8)),
-                     dim3(256), 0, Stream(), net->g2a, net->gc1, net->a[1],
-                     net->g1a, 8, 16, d.D1, d.H1, d.W1, d.P1 * 8);
+  hipLaunchKernelGGL(PoolBackSkipGateKernel, dim3(Blocks(d.P1)), dim3(256),
+                     0, Stream(), net->g2a, net->gc1, net->a[1], net->g1a, 8,
+                     16, d.D1, d.H1, d.W1, (int)d.P1);
   Check();
   LayerWgrad(net, 1, net->a[0], net->g1a, d.N, d.D1, d.H1, d.W1);
   LayerDgrad(net, 1, net->g1a, net->g1b, d.N, d.D1, d.H1, d.W1);
@@ -752,7 +858,7 @@ void FreeNet(__PSUNet *net) {
                               net->c2, net->c1, net->out, net->dout, net->g1a,
                               net->g1b, net->gc1, net->g2a, net->g2b,
                               net->gc2, net->g3a, net->g3b, net->part,
-                              net->bpart, net->lpart};
+                              net->bpart, net->tmp, net->lpart};
   for (int i = 0; i < 10; ++i) ptrs.push_back(net->a[i]);
   for (auto &L : net->layers) {
     ptrs.push_back(L.wt);
@@ -865,15 +971,20 @@ __PSUNet *PSUNetNew(int nx, int ny, int nzs, int max_batch, float out_scale,
   net->gc2 = A16(d.P2 * 48);
   net->g3a = A16(d.P3 * 32);
   net->g3b = A16(d.P3 * 32);
-  size_t pe = (size_t)kHeadBlocks * 9;
+  size_t pe = (size_t)kHeadBlocks * 9, be = 0, te = 9;
   for (int l = 0; l < 10; ++l) {
     const long P = l < 2 || l > 7 ? d.P1 : (l < 4 || l > 5 ? d.P2 : d.P3);
-    const size_t nb = (size_t)((P + kWgradChunk - 1) / kWgradChunk);
-    pe = std::max(pe, nb * net->layers[l].cout * 27 * net->layers[l].cin);
+    const long chunk = WgradChunk(P);
+    const size_t nb = (size_t)((P + chunk - 1) / chunk);
+    const size_t nw = (size_t)net->layers[l].cout * 27 * net->layers[l].cin;
+    pe = std::max(pe, nb * nw);
+    be = std::max(be, nb * net->layers[l].cout);
+    te = std::max(te, nw);
   }
   net->part_elems = pe;
   net->part = A32((long)pe);
-  net->bpart = A32((long)((d.P1 + kWgradChunk - 1) / kWgradChunk) * 32);
+  net->bpart = A32((long)be);
+  net->tmp = A32((long)(te * kSumSegments));
   net->lpart = (double *)Alloc(sizeof(double) * kLossBlocks);
   ok = ok && net->lpart;
   if (ok && hipHostMalloc((void **)&net->lhost, sizeof(double) * kLossBlocks) !=
@@ -1020,23 +1131,20 @@ void PSConv3dBackwardFilterHost(const uint16_t *x, const uint16_t *dy, int n,
                                 float *dw, float *db) {
   CheckLayerArgs(n, d, h, wd, cin, cout);
   const long P = (long)n * d * h * wd;
-  const long nb = (P + kWgradChunk - 1) / kWgradChunk;
+  const long nb = (P + WgradChunk(P) - 1) / WgradChunk(P);
   const size_t nw = (size_t)cout * 27 * cin;
   DevBuf ddx(P * cin * 2), ddy(P * cout * 2), part(nb * nw * 4),
-      bpart(nb * cout * 4), dw_d(nw * 4), db_d(cout * 4);
+      bpart(nb * cout * 4), dw_d(nw * 4), db_d(cout * 4),
+      tmp(nw * kSumSegments * 4);
   PS_HIP_CHECK(hipMemcpy(ddx.p, x, P * cin * 2, hipMemcpyHostToDevice));
   PS_HIP_CHECK(hipMemcpy(ddy.p, dy, P * cout * 2, hipMemcpyHostToDevice));
   int np = 0;
   Wgrad(cin, cout, (const u16 *)ddx.p, (const u16 *)ddy.p, (float *)part.p,
         (float *)bpart.p, n, d, h, wd, &np);
-  hipLaunchKernelGGL(SumPartsKernel, dim3(Blocks((long)nw)), dim3(256), 0,
-                     Stream(), (const float *)part.p, np, (long)nw,
-                     (float *)dw_d.p, (long)nw);
-  Check();
-  hipLaunchKernelGGL(SumPartsKernel, dim3(1), dim3(64), 0, Stream(),
-                     (const float *)bpart.p, np, (long)cout, (float *)db_d.p,
-                     (long)cout);
-  Check();
+  SumParts((const float *)part.p, np, (long)nw, (float *)tmp.p,
+           (float *)dw_d.p);
+  SumParts((const float *)bpart.p, np, (long)cout, (float *)tmp.p,
+           (float *)db_d.p);
   PS_HIP_CHECK(hipStreamSynchronize(Stream()));
   PS_HIP_CHECK(hipMemcpy(dw, dw_d.p, nw * 4, hipMemcpyDeviceToHost));
   PS_HIP_CHECK(hipMemcpy(db, db_d.p, cout * 4, hipMemcpyDeviceToHost));

@@ -232,7 +232,7 @@ def main(argv=None) -> int:
     ap.add_argument("--ntrain", type=int, default=256)
     ap.add_argument("--steps", type=int, default=1500)
     ap.add_argument("--batch", type=int, default=8)
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr", type=float, default=1e-3)
     a = ap.parse_args(argv)
     ps.PSInit()
     if a.what == "precision":

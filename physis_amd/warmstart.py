@@ -106,7 +106,7 @@ class UNet:
         return self.L.PSUNetTrainStep(self.handle, rhs.handle, bnd.handle,
                                       target.handle, first, count, lr)
 
-    def train(self, rhs, bnd, target, nproblems, steps, batch, lr=2e-3,
+    def train(self, rhs, bnd, target, nproblems, steps, batch, lr=1e-3,
               seed=0, lr_final=None, log=None):
         """Adam over random contiguous windows of `batch` training problems;
         lr decays linearly to lr_final.  Returns the loss history."""

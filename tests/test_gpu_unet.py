@@ -175,7 +175,7 @@ def test_warm_start_saves_sweeps(ps):
         scale = float(np.max(np.abs(sol)))
         net = UNet(nx, ny, nzs, 8, scale, seed=3)
         hist = net.train(grids[11], grids[10], tgt, ntrain, steps=800,
-                         batch=8, lr=5e-3, log=100)
+                         batch=8, lr=1e-3, log=100)
         first, last = np.mean(hist[:10]), np.mean(hist[-50:])
         print("loss first/last", first, last, flush=True)
         assert last < 0.5 * first
@@ -207,3 +207,55 @@ def test_warm_start_saves_sweeps(ps):
         solver.free()
         for g in grids + [p64]:
             ps.PSGridFree(g)
+
+
+def test_adam_steps_match_the_oracle(ps):
+    """Three training steps: each Adam update equals the one computed on the
+    host from the GPU's own gradient (FP32 Adam, 1e-6 relative), and the
+    gradient at the updated parameters still matches the oracle's."""
+    from physis_amd.warmstart import UNet
+    nx = ny = nzs = 32
+    nb, scale = 4, 2.0
+    shape = (nx, ny, nzs * nb)
+    coefs = poisson_problem(nx, ny, nzs, nbatch=nb, gaussians=(3, 6),
+                            sigma=(2.0, 6.0), seed=43)
+    rng = np.random.default_rng(8)
+    target = (rng.standard_normal(int(np.prod(shape))).astype(np.float32) *
+              coefs[10])
+    rhs, bnd = _problem_grids(ps, coefs, shape)
+    tg = ps.PSGridNew(shape)
+    ps.PSGridCopyin(tg, target)
+    net = UNet(nx, ny, nzs, 2, scale, seed=9)
+    m = v = None
+    m3 = nx * ny * nzs
+    try:
+        for step, first in enumerate((0, 2, 1), start=1):
+            p0 = net.get_params()
+            _, g = net.loss_grad(rhs, bnd, tg, first, 2)
+            lr = 1e-3
+            net.train_step(rhs, bnd, tg, first, 2, lr)
+            p1 = net.get_params()
+            g = g.astype(np.float32)
+            m = (np.float32(0.1) * g if m is None else
+                 np.float32(0.9) * m + np.float32(0.1) * g)
+            v = (np.float32(0.001) * g * g if v is None else
+                 np.float32(0.999) * v + np.float32(0.001) * g * g)
+            c1 = np.float32(1 - 0.9 ** step)
+            c2 = np.float32(1 - 0.999 ** step)
+            want = p0 - np.float32(lr) * (m / c1) / (np.sqrt(v / c2) +
+                                                     np.float32(1e-8))
+            d = np.max(np.abs(p1 - want))
+            print(f"step {step}: max |param - host Adam| {d:.3e}", flush=True)
+            assert d <= 1e-6 * max(1.0, float(np.max(np.abs(p0))))
+            _, g1 = net.loss_grad(rhs, bnd, tg, first, 2)
+            sl = slice(first * m3, (first + 2) * m3)
+            _, r1 = U.loss_grad(p1, coefs[11][sl], coefs[10][sl], target[sl],
+                                (nx, ny, nzs), 2, scale)
+            cos = float(g1 @ r1 / np.linalg.norm(g1) / np.linalg.norm(r1))
+            print(f"step {step}: gradient cosine vs oracle {cos:.6f}",
+                  flush=True)
+            assert cos >= 0.998
+    finally:
+        net.free()
+        for gg in (rhs, bnd, tg):
+            ps.PSGridFree(gg)
