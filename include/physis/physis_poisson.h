@@ -65,6 +65,15 @@ typedef struct {
   long sweeps;          /* Jacobi sweeps */
   double rel_residual;  /* max over problems of ||r|| / ||b||, FP64, at exit */
   double seconds;       /* wall time of the solve, device work included */
+  /* Optional trace, set by the caller (NULL / 0: none): after every
+   * residual evaluation the solver appends (sweeps, rel_residual, seconds)
+   * while history_len < history_cap. */
+  double *history;
+  int history_cap;
+  int history_len;
+  /* > 0: print (sweeps, residual, seconds) to stderr every that many
+   * seconds of solving, at residual checks (long solves). */
+  double progress_seconds;
 } PSPoissonStats;
 
 /* A solver over the 12 FP32 coefficient grids (a0 a1 a2 a3 b0 b1 b2 c0 c1 c2

@@ -82,7 +82,8 @@ def gaussian_rhs(nx: int, ny: int, nz: int, count: int, sig_lo: float,
 
 def poisson_problem(nx: int, ny: int, nzs: int, nbatch: int = 1,
                     noise: float = 0.01, gaussians=(16, 16),
-                    sigma=(4.0, 32.0), seed: int = 0):
+                    sigma=(4.0, 32.0), seed: int = 0,
+                    consistent: bool = False):
     """Coefficient grids (a0 a1 a2 a3 b0 b1 b2 c0 c1 c2 bnd wrk1) of nbatch
     Poisson problems of nx x ny x nzs stacked along z.
 
@@ -90,6 +91,14 @@ def poisson_problem(nx: int, ny: int, nzs: int, nbatch: int = 1,
     `noise` (b additive); bnd = 0 on the faces of every problem; wrk1 of
     problem q is a sum of gaussians[0] .. gaussians[1] Gaussians (the count
     drawn per problem) with widths in `sigma`, masked by bnd.
+
+    consistent=True reads "coefficients with +-noise" as a conductance
+    field: the six face coefficients a0..a2, c0..c2 carry the noise, b = 0
+    (a relative perturbation of 0), and a3 = 1 / (a0+a1+a2+c0+c1+c2), so
+    every row balances and the operator is a diagonally dominant M-matrix.
+    Perturbing a3 independently acts as a random potential of size ~noise
+    that makes the operator indefinite once its smallest eigenvalue
+    (~3 pi^2 / n^2) falls below it: with 5% noise Jacobi stalls at 128^3.
     """
     nz = nzs * nbatch
     n = nx * ny * nz
@@ -108,6 +117,12 @@ def poisson_problem(nx: int, ny: int, nzs: int, nbatch: int = 1,
         coefs.append((pct * nz_(4 + k)).astype(np.float32))
     for k in range(3):
         coefs.append((one * (one + pct * nz_(7 + k))).astype(np.float32))
+    if consistent:
+        tot = (coefs[0] + coefs[1] + coefs[2] + coefs[7] + coefs[8] +
+               coefs[9]).astype(np.float32)
+        coefs[3] = (one / tot).astype(np.float32)
+        for k in range(4, 7):
+            coefs[k] = np.zeros(n, np.float32)
     bnd = np.zeros((nbatch, nzs, ny, nx), np.float32)
     bnd[:, 1:-1, 1:-1, 1:-1] = 1.0
     bnd = bnd.reshape(-1)

@@ -538,10 +538,32 @@ void *FieldPtr(__PSPoisson *s, int field) {
   return nullptr;
 }
 
+// Appends (sweeps, rel, seconds since t0) to the caller's trace.
+void Trace(PSPoissonStats *st, double rel, double t0);
+
 double Now() {
   return std::chrono::duration<double>(
              std::chrono::steady_clock::now().time_since_epoch())
       .count();
+}
+
+void Trace(PSPoissonStats *st, double rel, double t0) {
+  if (st->progress_seconds > 0.0) {
+    static thread_local double last = 0.0;
+    const double now = Now();
+    if (st->sweeps == 0) last = now;
+    if (now - last >= st->progress_seconds) {
+      std::fprintf(stderr, "[physis-hip] solve: %ld sweeps, residual %.3e, "
+                   "%.1f s\n", st->sweeps, rel, now - t0);
+      last = now;
+    }
+  }
+  if (!st->history || st->history_len >= st->history_cap) return;
+  double *h = st->history + 3 * st->history_len;
+  h[0] = (double)st->sweeps;
+  h[1] = rel;
+  h[2] = Now() - t0;
+  st->history_len += 1;
 }
 
 }  // namespace
@@ -721,7 +743,15 @@ int PSPoissonSolveMixed(__PSPoisson *s, __PSGrid *p, float omega, double tol,
     PS_FATAL("PSPoissonSolveMixed: inner_sweeps must be even and >= 2");
   PSPoissonStats local{};
   if (!st) st = &local;
-  *st = PSPoissonStats{};
+  {
+    double *h = st->history;
+    const int cap = st->history_cap;
+    const double prog = st->progress_seconds;
+    *st = PSPoissonStats{};
+    st->history = h;
+    st->history_cap = cap;
+    st->progress_seconds = prog;
+  }
   if (!s->p2) {
     s->p2 = (double *)DevAlloc(s->elems * sizeof(double));
     if (!s->p2) PS_FATAL("PSPoissonSolveMixed: out of device memory");
@@ -735,6 +765,7 @@ int PSPoissonSolveMixed(__PSPoisson *s, __PSGrid *p, float omega, double tol,
   double rel, rmax;
   Residual(s, pa, true, nullptr, 0.0, 0.0, sums.data());
   Summarise(s, sums, &rel, &rmax);
+  Trace(st, rel, t0);
   double sigma = ScaleFor(rmax);  // scale of the RHS field
   bool rhs_ready = false;
   double best = rel;
@@ -759,6 +790,7 @@ int PSPoissonSolveMixed(__PSPoisson *s, __PSGrid *p, float omega, double tol,
     std::swap(pa, pb);
     st->outer += 1;
     Summarise(s, sums, &rel, &rmax);
+    Trace(st, rel, t0);
     if (rel < best * (1.0 - 1e-3)) {
       best = rel;
       no_gain = 0;
@@ -785,7 +817,15 @@ int PSPoissonSolvePlain(__PSPoisson *s, __PSGrid *p, float omega, double tol,
     PS_FATAL("PSPoissonSolvePlain: check must be even and >= 2");
   PSPoissonStats local{};
   if (!st) st = &local;
-  *st = PSPoissonStats{};
+  {
+    double *h = st->history;
+    const int cap = st->history_cap;
+    const double prog = st->progress_seconds;
+    *st = PSPoissonStats{};
+    st->history = h;
+    st->history_cap = cap;
+    st->progress_seconds = prog;
+  }
   if (!s->t32) {
     s->t32 = (float *)DevAlloc(s->elems * sizeof(float));
     if (!s->t32) PS_FATAL("PSPoissonSolvePlain: out of device memory");
@@ -796,6 +836,7 @@ int PSPoissonSolvePlain(__PSPoisson *s, __PSGrid *p, float omega, double tol,
   double rel, rmax;
   Residual(s, p->p, false, nullptr, 0.0, 0.0, sums.data());
   Summarise(s, sums, &rel, &rmax);
+  Trace(st, rel, t0);
   double best = rel;
   int no_gain = 0;
   while (rel > tol && st->sweeps < max_sweeps) {
@@ -804,6 +845,7 @@ int PSPoissonSolvePlain(__PSPoisson *s, __PSGrid *p, float omega, double tol,
     Residual(s, p->p, false, nullptr, 0.0, 0.0, sums.data());
     st->outer += 1;
     Summarise(s, sums, &rel, &rmax);
+    Trace(st, rel, t0);
     if (rel < best * (1.0 - 1e-3)) {
       best = rel;
       no_gain = 0;

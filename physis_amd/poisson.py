@@ -24,10 +24,48 @@ PS_POISSON_E, PS_POISSON_RHS, PS_POISSON_COEF0 = 0, 1, 2
 class PSPoissonStats(C.Structure):
     _fields_ = [("converged", C.c_int), ("stagnated", C.c_int),
                 ("outer", C.c_int), ("sweeps", C.c_long),
-                ("rel_residual", C.c_double), ("seconds", C.c_double)]
+                ("rel_residual", C.c_double), ("seconds", C.c_double),
+                ("history", C.POINTER(C.c_double)), ("history_cap", C.c_int),
+                ("history_len", C.c_int), ("progress_seconds", C.c_double)]
+
+    _SCALARS = ("converged", "stagnated", "outer", "sweeps", "rel_residual",
+                "seconds")
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        return {k: getattr(self, k) for k in self._SCALARS}
+
+
+def _stats(history: int, progress: float = 0.0):
+    st = PSPoissonStats()
+    st.progress_seconds = progress
+    buf = None
+    if history:
+        buf = np.zeros(3 * history, np.float64)
+        st.history = buf.ctypes.data_as(C.POINTER(C.c_double))
+        st.history_cap = history
+    return st, buf
+
+
+def _result(st, buf) -> dict:
+    d = st.as_dict()
+    if buf is not None:
+        d["history"] = buf[:3 * st.history_len].reshape(-1, 3).tolist()
+    return d
+
+
+def time_to(history, level: float):
+    """(sweeps, seconds) at which a trace first reached `level`, linearly
+    interpolated in log10(residual) between checks; None if never."""
+    prev = None
+    for sw, rel, t in history:
+        if rel <= level:
+            if prev is None or prev[1] <= level:
+                return sw, t
+            f = (np.log10(prev[1]) - np.log10(level)) / \
+                (np.log10(prev[1]) - np.log10(rel))
+            return prev[0] + f * (sw - prev[0]), prev[2] + f * (t - prev[2])
+        prev = (sw, rel, t)
+    return None
 
 
 _lib = None
@@ -93,18 +131,21 @@ class PoissonSolver:
             pass
 
     def solve_mixed(self, p, omega=0.8, tol=1e-6, inner=100,
-                    max_sweeps=10 ** 7) -> dict:
-        st = PSPoissonStats()
+                    max_sweeps=10 ** 7, history=0, progress=0.0) -> dict:
+        """Stats dict; with history=N also "history": up to N
+        (sweeps, rel_residual, seconds) rows, one per residual check;
+        progress > 0 prints a line to stderr about that often (seconds)."""
+        st, buf = _stats(history, progress)
         self.L.PSPoissonSolveMixed(self.handle, p.handle, omega, tol, inner,
                                    max_sweeps, C.byref(st))
-        return st.as_dict()
+        return _result(st, buf)
 
     def solve_plain(self, p, omega=0.8, tol=1e-6, check=100,
-                    max_sweeps=10 ** 7) -> dict:
-        st = PSPoissonStats()
+                    max_sweeps=10 ** 7, history=0, progress=0.0) -> dict:
+        st, buf = _stats(history, progress)
         self.L.PSPoissonSolvePlain(self.handle, p.handle, omega, tol, check,
                                    max_sweeps, C.byref(st))
-        return st.as_dict()
+        return _result(st, buf)
 
     def inner_sweeps(self, n: int, omega: float = 0.8) -> None:
         self.L.PSPoissonInnerSweeps(self.handle, n, omega)

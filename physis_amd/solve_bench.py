@@ -13,8 +13,9 @@ precision       the 19-point operator at n^3 (coefficients with 1% noise,
                 512^3 takes ~10^6 sweeps);
 precision-full  the same problem solved to 1e-6 at a size where that is
                 affordable (FP64-verified residual, sweeps, seconds);
-warmstart       64 problems of 128^3 (coefficients with 5% noise, 4..24
-                Gaussians, sigma 2..16): train the U-Net on `ntrain` other
+warmstart       64 problems of 128^3 (face coefficients with 5% noise, a
+                consistent diagonal, 4..24 Gaussians, sigma 2..16): train
+                the U-Net on `ntrain` other
                 problems solved by the Jacobi solve, then solve the 64 to
                 1e-6 from zero (plain FP32 and BF16 refinement) and from the
                 network's prediction.
@@ -117,7 +118,7 @@ def precision_full(n: int, tol: float = 1e-6) -> None:
     del coefs
     solver = PoissonSolver(grids, 1, "fp32")
     p = _grids([np.zeros(n ** 3, np.float32)], shape)[0]
-    st = solver.solve_plain(p, tol=tol, check=500)
+    st = solver.solve_plain(p, tol=tol, check=500, progress=30)
     _emit({"bench": "precision-full", "n": n, "storage": "plain_fp32", **st})
     base = st["seconds"] if st["converged"] else None
     solver.free()
@@ -125,7 +126,7 @@ def precision_full(n: int, tol: float = 1e-6) -> None:
     for storage in ("bf16", "fp16"):
         solver = PoissonSolver(grids, 1, storage)
         p = _grids([np.zeros(n ** 3)], shape, np.float64)[0]
-        st = solver.solve_mixed(p, tol=tol, inner=500)
+        st = solver.solve_mixed(p, tol=tol, inner=500, progress=30)
         if base:
             st["speedup_vs_plain_fp32"] = base / st["seconds"]
         _emit({"bench": "precision-full", "n": n,
@@ -139,7 +140,8 @@ def warmstart(ntrain: int, steps: int, batch: int, lr: float,
               tol: float = 1e-6, n: int = 128, ntest: int = 64) -> None:
     from .warmstart import UNet
     m = n ** 3
-    gen = dict(noise=0.05, gaussians=(4, 24), sigma=(2.0, 16.0))
+    gen = dict(noise=0.05, gaussians=(4, 24), sigma=(2.0, 16.0),
+               consistent=True)
     # training data: problems solved by the Jacobi solve (BF16 refinement),
     # 64 at a time
     chunk = 64
@@ -155,7 +157,7 @@ def warmstart(ntrain: int, steps: int, batch: int, lr: float,
         grids = _grids(coefs, shape)
         solver = PoissonSolver(grids, nb, "bf16")
         p = _grids([np.zeros(nb * m)], shape, np.float64)[0]
-        st = solver.solve_mixed(p, tol=1e-5, inner=500)
+        st = solver.solve_mixed(p, tol=1e-5, inner=500, progress=30)
         solve_s += st["seconds"]
         print(f"  training solutions {c0}..{c0 + nb}: {st}", flush=True)
         rhs_all[c0 * m:(c0 + nb) * m] = coefs[11]
@@ -184,43 +186,43 @@ def warmstart(ntrain: int, steps: int, batch: int, lr: float,
     shape = (n, n, n * ntest)
     grids = _grids(coefs, shape)
     res = {}
+    H = 8192
     solver = PoissonSolver(grids, ntest, "fp32")
     p = _grids([np.zeros(ntest * m, np.float32)], shape)[0]
-    res["plain_fp32_zero"] = solver.solve_plain(p, tol=tol, check=500)
+    res["plain_fp32_zero"] = solver.solve_plain(p, tol=tol, check=500,
+                                                history=H, progress=30)
     solver.free()
     _free([p])
     solver = PoissonSolver(grids, ntest, "bf16")
     p = _grids([np.zeros(ntest * m)], shape, np.float64)[0]
-    res["mixed_bf16_zero"] = solver.solve_mixed(p, tol=tol, inner=500)
+    res["mixed_bf16_zero"] = solver.solve_mixed(p, tol=tol, inner=500,
+                                                history=H, progress=30)
     # warm start: prediction (timed) then the same solve
     t0 = time.perf_counter()
     for f in range(0, ntest, batch):
         net.predict(grids[11], grids[10], f, min(batch, ntest - f), p)
     t_pred = time.perf_counter() - t0
-    st = solver.solve_mixed(p, tol=tol, inner=500)
+    st = solver.solve_mixed(p, tol=tol, inner=500, history=H, progress=30)
     st["predict_seconds"] = t_pred
-    st["seconds_total"] = st["seconds"] + t_pred
     res["mixed_bf16_warm"] = st
-    # the network alone + plain FP32 Jacobi
     solver.free()
-    solver = PoissonSolver(grids, ntest, "fp32")
-    p32 = _grids([np.zeros(ntest * m, np.float32)], shape)[0]
-    t0 = time.perf_counter()
-    for f in range(0, ntest, batch):
-        net.predict(grids[11], grids[10], f, min(batch, ntest - f), p32)
-    t_pred = time.perf_counter() - t0
-    st = solver.solve_plain(p32, tol=tol, check=500)
-    st["predict_seconds"] = t_pred
-    st["seconds_total"] = st["seconds"] + t_pred
-    res["plain_fp32_warm"] = st
-    solver.free()
-    base = res["plain_fp32_zero"]["seconds"]
+    # equal-accuracy comparisons: at 1e-6, and at the residual plain FP32
+    # from zero reached (it may stop above 1e-6)
+    from .poisson import time_to
+    base = res["plain_fp32_zero"]
+    levels = sorted({tol, max(base["rel_residual"], tol)})
     for k, v in res.items():
-        tot = v.get("seconds_total", v["seconds"])
-        _emit({"bench": "warmstart", "problems": ntest, "n": n, "run": k,
-               "speedup_vs_plain_fp32_zero": base / tot, **v})
+        hist = v.pop("history")
+        out = {"bench": "warmstart", "problems": ntest, "n": n, "run": k,
+               **v, "initial_rel_residual": hist[0][1] if hist else None}
+        for lv in levels:
+            t = time_to(hist, lv)
+            if t is not None:
+                pre = v.get("predict_seconds", 0.0)
+                out[f"to_{lv:.3g}"] = {"sweeps": t[0], "seconds": t[1] + pre}
+        _emit(out)
     net.free()
-    _free(grids + [p, p32])
+    _free(grids + [p])
 
 
 def main(argv=None) -> int:
