@@ -68,7 +68,8 @@ void LoadConfig(const std::string &path, Config *cfg) {
     else if (k == "HIP_KERNEL_ERROR_CHECK") cfg->kernel_error_check = on;
     else if (k == "TRACE_KERNEL") cfg->trace = on;
   }
-  if (cfg->prefetch < 2 || cfg->prefetch % 2) cfg->prefetch = 2;
+  if (cfg->prefetch != 0 && (cfg->prefetch < 2 || cfg->prefetch % 2))
+    cfg->prefetch = 2;
   int bw = cfg->block_waves;
   if (bw != 0 && bw != 4 && bw != 8 && bw != 16) cfg->block_waves = 0;
 }

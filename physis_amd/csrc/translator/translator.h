@@ -26,7 +26,7 @@ namespace physisc {
 struct Config {
   // Launch shapes of the 2.5D run kernel (waves per block, prefetch depth).
   int block_waves = 0;  // 0: chosen from the LDS budget
-  int prefetch = 2;
+  int prefetch = 0;     // 0: chosen from the register estimate (2 or 4)
   bool kernel_error_check = false;  // HIP_KERNEL_ERROR_CHECK
   bool trace = false;               // TRACE_KERNEL
 };
@@ -110,6 +110,10 @@ class Translator {
   std::string EmitFastRunKernel(const KernelInfo &k);
   std::string EmitGenericRunKernel(const KernelInfo &k);
   std::string EmitLauncher(const KernelInfo &k);
+  // VGPRs per lane of the run kernel at VX = 4 with pd staged planes in
+  // flight (a rough estimate), and the prefetch depth used for k.
+  int RegEstimate(const KernelInfo &k, int pd) const;
+  int PrefetchFor(const KernelInfo &k) const;
   std::string EmitRunFunction(const RunSite &r);
 
   // host code

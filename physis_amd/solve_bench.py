@@ -90,8 +90,8 @@ def precision(n: int, sweeps: int, tol: float = 1e-6) -> None:
     for storage in ("bf16", "fp16"):
         solver = PoissonSolver(grids, 1, storage)
         p = _grids([np.zeros(n ** 3)], shape, np.float64)[0]
-        st = solver.solve_mixed(p, tol=tol, inner=500, max_sweeps=sweeps // 2)
-        st2 = solver.solve_mixed(p, tol=tol, inner=500, max_sweeps=sweeps // 2)
+        st = solver.solve_mixed(p, tol=tol, inner=200, max_sweeps=sweeps // 2)
+        st2 = solver.solve_mixed(p, tol=tol, inner=200, max_sweeps=sweeps // 2)
         per = st2["seconds"] / st2["sweeps"]
         rate = _rate(st["rel_residual"], st2["rel_residual"], st2["sweeps"])
         need = (math.log10(tol) - math.log10(st2["rel_residual"])) / rate \
@@ -126,7 +126,7 @@ def precision_full(n: int, tol: float = 1e-6) -> None:
     for storage in ("bf16", "fp16"):
         solver = PoissonSolver(grids, 1, storage)
         p = _grids([np.zeros(n ** 3)], shape, np.float64)[0]
-        st = solver.solve_mixed(p, tol=tol, inner=500, progress=30)
+        st = solver.solve_mixed(p, tol=tol, inner=200, progress=30)
         if base:
             st["speedup_vs_plain_fp32"] = base / st["seconds"]
         _emit({"bench": "precision-full", "n": n,
@@ -157,7 +157,7 @@ def warmstart(ntrain: int, steps: int, batch: int, lr: float,
         grids = _grids(coefs, shape)
         solver = PoissonSolver(grids, nb, "bf16")
         p = _grids([np.zeros(nb * m)], shape, np.float64)[0]
-        st = solver.solve_mixed(p, tol=1e-5, inner=500, progress=30)
+        st = solver.solve_mixed(p, tol=1e-5, inner=200, progress=30)
         solve_s += st["seconds"]
         print(f"  training solutions {c0}..{c0 + nb}: {st}", flush=True)
         rhs_all[c0 * m:(c0 + nb) * m] = coefs[11]
@@ -195,14 +195,14 @@ def warmstart(ntrain: int, steps: int, batch: int, lr: float,
     _free([p])
     solver = PoissonSolver(grids, ntest, "bf16")
     p = _grids([np.zeros(ntest * m)], shape, np.float64)[0]
-    res["mixed_bf16_zero"] = solver.solve_mixed(p, tol=tol, inner=500,
+    res["mixed_bf16_zero"] = solver.solve_mixed(p, tol=tol, inner=200,
                                                 history=H, progress=30)
     # warm start: prediction (timed) then the same solve
     t0 = time.perf_counter()
     for f in range(0, ntest, batch):
         net.predict(grids[11], grids[10], f, min(batch, ntest - f), p)
     t_pred = time.perf_counter() - t0
-    st = solver.solve_mixed(p, tol=tol, inner=500, history=H, progress=30)
+    st = solver.solve_mixed(p, tol=tol, inner=200, history=H, progress=30)
     st["predict_seconds"] = t_pred
     res["mixed_bf16_warm"] = st
     solver.free()
