@@ -6,7 +6,9 @@ Run under the profiler, e.g.
 It issues, in order: a device-to-device copy of one 512 MiB grid (the
 runtime's plain copy kernel, whose FETCH_SIZE / WRITE_SIZE calibrate the
 counters' units), 10 sweeps of the 7-point clamped and periodic run kernels
-at 512^3, 10 Himeno sweeps at 512x256x256, and 3 PSReduce SUMs at 512^3.
+at 512^3, 3 PSReduce SUMs at 512^3, 10 Himeno sweeps at 512x256x256, and
+the precision tier at 512^3: 10 BF16 inner sweeps and 3 fused FP64
+update + residual passes.
 Then `python3 -m physis_amd.profile_summary OUT_fetch OUT_write` writes the
 per-kernel HBM traffic summary.
 """
@@ -52,6 +54,27 @@ def main() -> None:
         grids.append(g)
     Himeno().run(10, grids[12], grids[13], grids[:12])
     for g in grids:
+        ps.PSGridFree(g)
+    # precision tier at 512^3: 10 BF16 inner sweeps, 3 fused FP64
+    # update + residual passes
+    from .configs import poisson_problem
+    from .poisson import PoissonSolver
+    coefs = poisson_problem(*shape, nbatch=1, gaussians=(16, 16),
+                            sigma=(4.0, 32.0), seed=0)
+    grids = []
+    for c in coefs:
+        g = ps.PSGridNew(shape)
+        ps.PSGridCopyin(g, c)
+        grids.append(g)
+    del coefs
+    solver = PoissonSolver(grids, 1, "bf16")
+    solver.inner_sweeps(10)
+    p = ps.PSGridNew(shape, np.float64)
+    q = ps.PSGridNew(shape, np.float64)
+    for _ in range(3):
+        solver.residual(p, q, 0.5, 1.0)
+    solver.free()
+    for g in grids + [p, q]:
         ps.PSGridFree(g)
     ps.PSFinalize()
     print("profile workload done", float(np.float32(0)))

@@ -28,6 +28,11 @@ KERNELS = [  # (label, name fragment, algorithmic bytes per launch)
     ("19-pt Himeno run kernel, 512x256x256", "__PSStencilRun_jacobi_kernel<",
      56 * 512 * 256 * 256),
     ("PSReduce SUM stage 1, 512^3", "ReduceStage1<float, 2>", 4 * 512 ** 3),
+    ("BF16 inner Jacobi sweep, 512^3", "SweepKernel<physis_hip::bf16_t",
+     28 * 512 ** 3),
+    ("FP64 update + residual (BF16 E/RHS), 512^3",
+     "ResidualKernel<double, physis_hip::bf16_t, 2, true, true>",
+     68 * 512 ** 3),
 ]
 
 
