@@ -282,6 +282,257 @@ void Translator::AnalyzeKernel(KernelInfo &k) {
   }
   AnalyzeGets(k, *f.body);
   Decide(k);
+  AnalyzeFusion(k);
+}
+
+// ------------------------------------------------- two-sweep fusion analysis
+// A map pair k(a -> b), k(b -> a) over the whole grid can run as one pass
+// that computes u' = k(a) on a tile plus a one-point halo and u'' = k(u')
+// on the tile, when the result is exactly the two sweeps': k reads one grid
+// (staged, halo <= 1) and emits the other unconditionally, and the u' values
+// it computes at halo points outside the grid are never used -- either every
+// neighbour get is periodic and the index parameters appear only in get /
+// emit indices (so k at x = -1 equals k at x = n-1), or every neighbour get
+// at offset -1 / +1 on an axis sits in the else branch of a test
+// `idx == 0` / `idx == n - 1` with n = PSGridDim(grid, axis) (the clamped
+// boundary of the reference diffusion benchmark, diffusion3d_physis.c:
+// 40-51).
+namespace {
+
+const Expr *Unparen(const Expr *e) {
+  while (e && e->kind == EK::kParen) e = e->kids[0].get();
+  return e;
+}
+
+bool IsAssignOp(const std::string &op) {
+  return op == "=" || op == "+=" || op == "-=" || op == "*=" || op == "/=" ||
+         op == "%=" || op == "&=" || op == "|=" || op == "^=" ||
+         op == "<<=" || op == ">>=";
+}
+
+struct FuseScan {
+  const KernelInfo &k;
+  std::map<std::string, int> assigns;   // assignments per identifier
+  std::map<std::string, int> dim_axis;  // identifier = PSGridDim(g, axis)
+  bool ok = true;
+  std::string why;
+  int emits = 0;
+
+  explicit FuseScan(const KernelInfo &kk) : k(kk) {}
+
+  void Fail(const std::string &w) {
+    if (ok) why = w;
+    ok = false;
+  }
+  static bool DimOf(const Expr &e, int *axis) {
+    const Expr *x = Unparen(&e);
+    const std::string *c = CalleeName(*x);
+    int a;
+    if (!c || *c != "PSGridDim" || x->kids.size() != 3 ||
+        !ConstInt(*x->kids[2], &a))
+      return false;
+    *axis = a;
+    return true;
+  }
+  // assignments and dimension variables
+  void Assigns(const Expr &e) {
+    if (e.kind == EK::kBinary && IsAssignOp(e.op)) {
+      const Expr *l = Unparen(e.kids[0].get());
+      if (l->kind == EK::kIdent) {
+        assigns[l->text]++;
+        int a;
+        if (e.op == "=" && DimOf(*e.kids[1], &a)) dim_axis[l->text] = a;
+      }
+    }
+    if ((e.kind == EK::kUnary || e.kind == EK::kPostfix) &&
+        (e.op == "++" || e.op == "--")) {
+      const Expr *l = Unparen(e.kids[0].get());
+      if (l->kind == EK::kIdent) assigns[l->text] += 2;
+    }
+    for (const auto &c : e.kids)
+      if (c) Assigns(*c);
+  }
+  void Assigns(const Stmt &s) {
+    for (const auto &d : s.decls)
+      if (d.init) {
+        assigns[d.name]++;
+        int a;
+        if (DimOf(*d.init, &a)) dim_axis[d.name] = a;
+        Assigns(*d.init);
+      }
+    for (const Expr *e : {s.expr.get(), s.cond.get(), s.step.get()})
+      if (e) Assigns(*e);
+    if (s.init) Assigns(*s.init);
+    for (const auto &c : s.kids)
+      if (c) Assigns(*c);
+  }
+  int Axis(const std::string &n) const {
+    for (int a = 0; a < (int)k.index.size(); ++a)
+      if (k.index[a] == n) return a;
+    return -1;
+  }
+  // `idx_a == 0` -> (a, 0); `idx_a == n - 1` with n a dimension variable of
+  // axis a (assigned once) or PSGridDim(g, a) - 1 -> (a, 1)
+  bool Guard(const Expr &cond, std::pair<int, int> *g) const {
+    const Expr *c = Unparen(&cond);
+    if (c->kind != EK::kBinary || c->op != "==") return false;
+    for (int side = 0; side < 2; ++side) {
+      const Expr *l = Unparen(c->kids[side].get());
+      const Expr *r = Unparen(c->kids[1 - side].get());
+      if (l->kind != EK::kIdent) continue;
+      const int a = Axis(l->text);
+      if (a < 0) continue;
+      int v;
+      if (ConstInt(*r, &v) && v == 0) {
+        *g = {a, 0};
+        return true;
+      }
+      if (r->kind == EK::kBinary && r->op == "-" && ConstInt(*r->kids[1], &v) &&
+          v == 1) {
+        const Expr *n = Unparen(r->kids[0].get());
+        int da;
+        if (n->kind == EK::kIdent && dim_axis.count(n->text) &&
+            assigns.at(n->text) == 1 && dim_axis.at(n->text) == a) {
+          *g = {a, 1};
+          return true;
+        }
+        if (DimOf(*n, &da) && da == a) {
+          *g = {a, 1};
+          return true;
+        }
+      }
+    }
+    return false;
+  }
+  // gets of the read grid must be guarded on every axis they step along
+  void Gets(const Expr &e, std::vector<std::pair<int, int>> &facts,
+            bool periodic) {
+    if (e.kind == EK::kTernary && !periodic) {
+      Gets(*e.kids[0], facts, periodic);
+      std::pair<int, int> g;
+      if (Guard(*e.kids[0], &g)) {
+        Gets(*e.kids[1], facts, periodic);
+        facts.push_back(g);
+        Gets(*e.kids[2], facts, periodic);
+        facts.pop_back();
+        return;
+      }
+    }
+    if (const std::string *c = CalleeName(e)) {
+      if (*c == "PSGridEmit") {
+        emits++;
+        for (size_t i = 2; i < e.kids.size(); ++i) Gets(*e.kids[i], facts, periodic);
+        return;
+      }
+      if (*c == "PSGridGet" || *c == "PSGridGetPeriodic") {
+        if (!periodic) {
+          std::array<int, 3> o = {0, 0, 0};
+          for (int a = 0; a < 3; ++a)
+            if (!RegularIndexOf(k, *e.kids[2 + a], a, &o[a]))
+              return Fail("an irregular get");
+          for (int a = 0; a < 3; ++a) {
+            if (!o[a]) continue;
+            bool have = false;
+            for (auto &f : facts)
+              have = have || (f.first == a && f.second == (o[a] < 0 ? 0 : 1));
+            if (!have) return Fail("a neighbour get without a boundary guard");
+          }
+        }
+        return;  // index arguments are get indices
+      }
+      if (*c == "PSGridDim") return;
+      if (c->compare(0, 2, "PS") != 0) return Fail("a call of " + *c);
+    }
+    if (periodic && e.kind == EK::kIdent && Axis(e.text) >= 0)
+      return Fail("an index parameter used outside get / emit indices");
+    for (const auto &kid : e.kids)
+      if (kid) Gets(*kid, facts, periodic);
+  }
+  void Gets(const Stmt &s, std::vector<std::pair<int, int>> &facts,
+            bool periodic) {
+    if (s.kind == SK::kIf) {
+      Gets(*s.expr, facts, periodic);
+      std::pair<int, int> g;
+      const bool guard = !periodic && Guard(*s.expr, &g);
+      if (!s.kids.empty() && s.kids[0]) Gets(*s.kids[0], facts, periodic);
+      if (s.kids.size() > 1 && s.kids[1]) {
+        if (guard) facts.push_back(g);
+        Gets(*s.kids[1], facts, periodic);
+        if (guard) facts.pop_back();
+      }
+      return;
+    }
+    for (const auto &d : s.decls)
+      if (d.init) Gets(*d.init, facts, periodic);
+    for (const Expr *e : {s.expr.get(), s.cond.get(), s.step.get()})
+      if (e) Gets(*e, facts, periodic);
+    if (s.init) Gets(*s.init, facts, periodic);
+    for (const auto &c : s.kids)
+      if (c) Gets(*c, facts, periodic);
+  }
+  static bool RegularIndexOf(const KernelInfo &k, const Expr &e, int axis,
+                             int *off) {
+    const Expr *x = Unparen(&e);
+    const std::string &idx = k.index[axis];
+    if (x->kind == EK::kIdent && x->text == idx) {
+      *off = 0;
+      return true;
+    }
+    if (x->kind == EK::kBinary && (x->op == "+" || x->op == "-")) {
+      const Expr *a = Unparen(x->kids[0].get());
+      int c;
+      if (a->kind == EK::kIdent && a->text == idx && ConstInt(*x->kids[1], &c)) {
+        *off = x->op == "+" ? c : -c;
+        return true;
+      }
+    }
+    return false;
+  }
+};
+
+}  // namespace
+
+void Translator::AnalyzeFusion(KernelInfo &k) {
+  k.fusable = false;
+  auto no = [&](const std::string &w) { k.fuse_why = w; };
+  if (!k.fast || k.ndims != 3) return no("not a 2.5D run kernel");
+  if (k.ret_type != "void") return no("a reduction kernel");
+  if (k.grids.size() != 2) return no("not exactly two grids");
+  int in = -1, out = -1;
+  for (int i = 0; i < 2; ++i) {
+    const GridParam &g = k.grids[i];
+    if (g.read && !g.emitted) in = i;
+    if (g.emitted && !g.read) out = i;
+  }
+  if (in < 0 || out < 0) return no("not one read grid and one emitted grid");
+  const GridParam &g = k.grids[in];
+  if (g.mode != GridParam::kStaged || g.irregular)
+    return no("the read grid is not staged");
+  if (g.elem != "float" || k.grids[out].elem != "float")
+    return no("not float grids");
+  if (g.hx > 1 || g.hy > 1 || g.zl < -1 || g.zr > 1)
+    return no("halo wider than one point");
+  for (size_t i = 0; i < g.offsets.size(); ++i) {
+    const auto &o = g.offsets[i];
+    if ((o[0] || o[1] || o[2]) && g.offset_periodic[i] != g.periodic)
+      return no("periodic and non-periodic neighbour gets mixed");
+  }
+  FuseScan scan(k);
+  scan.Assigns(*k.fn->body);
+  std::vector<std::pair<int, int>> facts;
+  scan.Gets(*k.fn->body, facts, g.periodic);
+  if (!scan.ok) return no(scan.why);
+  if (scan.emits != 1) return no("not exactly one emit");
+  bool top = false;
+  for (const auto &st : k.fn->body->kids)
+    if (st && st->kind == SK::kExpr && st->expr) {
+      const std::string *c = CalleeName(*Unparen(st->expr.get()));
+      top = top || (c && *c == "PSGridEmit");
+    }
+  if (!top) return no("the emit is conditional");
+  k.fusable = true;
+  k.fuse_in = in;
+  k.fuse_out = out;
 }
 
 void Translator::Decide(KernelInfo &k) {
@@ -463,6 +714,9 @@ std::string Translator::EmitContext(const KernelInfo &k) {
       o << "\n";
     }
   }
+  if (k.fast)
+    o << "//   two-sweep fusion of map pairs: "
+      << (k.fusable ? std::string("yes") : "no (" + k.fuse_why + ")") << "\n";
   o << "struct __PSCtx_" << k.name << " {\n";
   for (auto &g : k.grids) {
     if (g.mode == GridParam::kStaged && k.fast)
@@ -779,6 +1033,197 @@ std::string Translator::EmitGenericRunKernel(const KernelInfo &k) {
   return o.str();
 }
 
+// Two-sweep run kernel (temporal blocking) of a fusable kernel, its launcher
+// and the run-time check that a map pair may use it.  Block: BY waves, wave
+// r holds source row y0 - 2 + r; ring A (LDS) holds source planes s-1..s+1
+// with a two-point x halo, ring B the intermediate planes s-3..s-1 with a
+// one-point halo.  Step s computes u'(s) on rows y0-1 .. y0+TY into ring B
+// (the halo columns x0-1 and x0+TX by the lanes of wave 0, which otherwise
+// only loads), and u''(s-2) from ring B on the tile's rows; one barrier per
+// step.
+std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
+  std::ostringstream o;
+  const std::string &n = k.name;
+  const GridParam &gi = k.grids[k.fuse_in], &go = k.grids[k.fuse_out];
+  const std::string T = gi.elem, V = gi.dev_view, I = gi.name, O = go.name;
+  const std::string per = gi.periodic ? "true" : "false";
+  std::vector<std::string> sp, sa;
+  for (auto &sc : k.scalars) {
+    sp.push_back(sc.type + " " + sc.name);
+    sa.push_back(sc.name);
+  }
+  std::string fargs = "ctx";
+  for (auto &a : sa) fargs += ", " + a;
+  auto rows = RowsOf(gi);
+  // ctx fill of the lane's point i from row arrays <pre>_r<r>
+  auto fill = [&](const std::string &pre, const std::string &ind) {
+    std::ostringstream f;
+    for (size_t i = 0; i < gi.offsets.size(); ++i) {
+      const auto &off = gi.offsets[i];
+      for (size_t r = 0; r < rows.size(); ++r)
+        if (rows[r].dy == off[1] && rows[r].dz == off[2])
+          f << ind << "ctx." << I << "[" << i << "] = " << pre << "_r" << r
+            << "[i + " << off[0] - rows[r].xl << "];\n";
+    }
+    return f.str();
+  };
+  auto read_rows = [&](const std::string &pre, const std::string &lds,
+                       const std::string &S, const std::string &base,
+                       const std::string &ind) {
+    std::ostringstream f;
+    for (size_t r = 0; r < rows.size(); ++r) {
+      const Row &w = rows[r];
+      f << ind << T << " " << pre << "_r" << r << "[VX + " << w.xr - w.xl
+        << "];\n";
+      f << ind << S << "::template ReadRow<" << w.xl << ", " << w.xr << ">("
+        << lds << " + physis_hip::RingSlot<4>(" << base << ", " << w.dz + 1
+        << ") * " << S << "::kPlane, wave + " << w.dy << ", lane, " << pre
+        << "_r" << r << ");\n";
+    }
+    return f.str();
+  };
+  o << "// Two-sweep run kernel of " << n
+    << " (temporal blocking): one pass over src\n"
+       "// computes u' = " << n << "(src) on the tile and a one-point halo and\n"
+       "// u'' = " << n << "(u') two planes behind; dst <- u'', mid <- u' when\n"
+       "// mid is not NULL.\n";
+  o << "template <int BY>\n__global__ __launch_bounds__(64 * BY) void __PSStencilRun2_"
+    << n << "(\n    __PSDomain dom, " << V << " src, " << V << " dst, " << T
+    << " *__restrict__ mid" << (sp.empty() ? "" : ", " + Join(sp, ", "))
+    << ", physis_hip::RunShape sh) {\n";
+  o << "  constexpr int VX = 4, PD = 2, TX = 64 * VX, TY = BY - 4;\n"
+    << "  typedef physis_hip::Stage<" << T << ", VX, BY, 2, 2, 4, " << per
+    << "> SA;\n"
+    << "  typedef physis_hip::Stage<" << T << ", VX, BY, 1, 1, 4, false> SB;\n"
+    << "  __shared__ __attribute__((aligned(16))) " << T << " a_lds[SA::kLdsElems];\n"
+    << "  __shared__ __attribute__((aligned(16))) " << T << " b_lds[SB::kLdsElems];\n"
+    << "  const int lane = threadIdx.x % 64;\n"
+       "  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);\n"
+       "  const physis_hip::TileCoord tc = physis_hip::TileOf(sh.tiles_x, sh.tiles_y);\n"
+       "  const int x0 = (sh.x_first_tile + tc.bx) * TX;\n"
+       "  const int y0 = dom.local_min[1] + tc.by * TY;\n"
+       "  const int zb = dom.local_min[2] + tc.bz * sh.zchunk;\n"
+       "  const int ze = min(zb + sh.zchunk, dom.local_max[2]);\n"
+       "  if (zb >= ze) return;  // uniform over the block\n"
+       "  const int yrow = y0 - 2 + wave;\n"
+       "  const bool mid_row = wave >= 1 && wave < BY - 1;\n"
+       "  const bool out_row = wave >= 2 && wave < BY - 2 && yrow < dom.local_max[1];\n"
+       "  const int xs = x0 + lane * VX;\n"
+       "  const int nx = src.dim[0], nxy = src.dim[0] * src.dim[1];\n"
+       "  const int n = nxy * src.dim[2];\n"
+    << "  const physis_hip::BufWin<" << T << "> aw = physis_hip::BufWin<" << T
+    << ">::Make(src.p, n, (long)(zb - 3) * nxy, (long)(ze + 3) * nxy);\n"
+    << "  const physis_hip::BufWin<" << T << "> dw = physis_hip::BufWin<" << T
+    << ">::Make(dst.p, n, (long)zb * nxy, (long)ze * nxy);\n"
+    << "  const physis_hip::BufWin<" << T << "> mw = physis_hip::BufWin<" << T
+    << ">::Make(mid ? mid : dst.p, n, (long)zb * nxy, (long)ze * nxy);\n"
+    << "  SA st[PD];\n"
+       "#pragma unroll\n  for (int k = 0; k < 3; ++k) {\n"
+       "    st[0].Load(aw, src.p, src.dim, x0, yrow, zb - 2 + k, lane);\n"
+       "    st[0].Store(a_lds + k * SA::kPlane, wave, lane);\n  }\n"
+       "#pragma unroll\n  for (int k = 0; k < PD; ++k)\n"
+       "    st[k].Load(aw, src.p, src.dim, x0, yrow, zb + 1 + k, lane);\n"
+       "  int abase = 0, bbase = 0;\n"
+       "  __syncthreads();\n"
+    << "  __PSCtx_" << n << " ctx;\n"
+    << "  ctx." << I << "_p = src.p;\n  ctx." << O << "_p = dst.p;\n"
+    << "  for (int d = 0; d < 3; ++d) {\n    ctx." << I << "_dim[d] = src.dim[d];\n"
+    << "    ctx." << O << "_dim[d] = dst.dim[d];\n  }\n"
+    << "  for (int sc = zb - 1; sc <= ze + 1; sc += PD) {\n#pragma unroll\n"
+       "    for (int k = 0; k < PD; ++k) {\n"
+       "      const int s = sc + k;\n"
+       "      if (s > ze + 1) break;  // uniform over the block\n"
+       "      if (mid_row && s <= ze) {\n";
+  o << read_rows("a", "a_lds", "SA", "abase", "        ");
+  o << "        " << T << " uo[VX];\n"
+       "#pragma unroll\n        for (int i = 0; i < VX; ++i) {\n";
+  o << fill("a", "          ");
+  o << "          ctx." << O << "_e = false;\n"
+    << "          " << n << "(xs + i, yrow, s, " << fargs << ");\n"
+    << "          uo[i] = ctx." << O << "_v;\n        }\n";
+  o << "        " << T << " *brow = b_lds + physis_hip::RingSlot<4>(bbase, 3) * "
+       "SB::kPlane + wave * SB::kRow + SB::kPad;\n"
+       "        *reinterpret_cast<physis_hip::NativeVec<" << T
+    << ", 4>::V *>(brow + lane * VX) =\n            physis_hip::NativeVec<" << T
+    << ", 4>::V{uo[0], uo[1], uo[2], uo[3]};\n";
+  o << "        if (mid && out_row && s >= zb && s < ze) {\n"
+       "          unsigned m = 0;\n"
+       "#pragma unroll\n          for (int i = 0; i < VX; ++i)\n"
+       "            if (xs + i >= dom.local_min[0] && xs + i < dom.local_max[0]) m |= 1u << i;\n"
+    << "          physis_hip::StoreMasked<" << T
+    << ", VX>(mw, xs + yrow * nx + s * nxy, uo, m);\n        }\n      }\n";
+  // the halo columns x0 - 1 and x0 + TX of u' rows 1 .. BY-2, one point
+  // per lane of wave 0 (which loads only): lane 2(r-1) + c, column c
+  o << "      if (wave == 0 && lane < 2 * (BY - 2) && s <= ze) {\n"
+       "        const int hr = 1 + (lane >> 1);\n"
+       "        const int px = (lane & 1) ? TX : -1;\n";
+  for (size_t i = 0; i < gi.offsets.size(); ++i) {
+    const auto &off = gi.offsets[i];
+    o << "        ctx." << I << "[" << i << "] = a_lds[physis_hip::RingSlot<4>(abase, "
+      << off[2] + 1 << ") * SA::kPlane + (hr + " << off[1]
+      << ") * SA::kRow + SA::kPad + px + " << off[0] << "];\n";
+  }
+  o << "        ctx." << O << "_e = false;\n"
+    << "        " << n << "(x0 + px, y0 - 2 + hr, s, " << fargs << ");\n"
+    << "        b_lds[physis_hip::RingSlot<4>(bbase, 3) * SB::kPlane + hr * SB::kRow "
+       "+ SB::kPad + px] = ctx." << O << "_v;\n      }\n";
+  o << "      if (out_row && s - 2 >= zb) {\n";
+  o << read_rows("b", "b_lds", "SB", "bbase", "        ");
+  o << "        " << T << " uu[VX];\n        unsigned m = 0;\n"
+       "#pragma unroll\n        for (int i = 0; i < VX; ++i) {\n"
+       "          uu[i] = 0;\n"
+       "          if (xs + i >= dom.local_min[0] && xs + i < dom.local_max[0]) {\n";
+  o << fill("b", "            ");
+  o << "            ctx." << O << "_e = false;\n"
+    << "            " << n << "(xs + i, yrow, s - 2, " << fargs << ");\n"
+    << "            uu[i] = ctx." << O << "_v;\n"
+    << "            m |= 1u << i;\n          }\n        }\n"
+    << "        physis_hip::StoreMasked<" << T
+    << ", VX>(dw, xs + yrow * nx + (s - 2) * nxy, uu, m);\n      }\n"
+    << "      st[k].Store(a_lds + physis_hip::RingSlot<4>(abase, 3) * SA::kPlane, "
+       "wave, lane);\n"
+       "      __syncthreads();\n"
+       "      st[k].Load(aw, src.p, src.dim, x0, yrow, s + 2 + PD, lane);\n"
+       "      abase = physis_hip::RingSlot<4>(abase, 1);\n"
+       "      bbase = physis_hip::RingSlot<4>(bbase, 1);\n"
+       "    }\n  }\n}\n\n";
+  // launcher
+  std::vector<std::string> largs = {"s.dom", "src", "dst", "mid"};
+  for (auto &a : sa) largs.push_back("s." + a);
+  largs.push_back("sh");
+  o << "static void __PSLaunch2_" << n << "(const struct __PSStencil_" << n
+    << " &s, const " << V << " &src, const " << V << " &dst, " << T
+    << " *mid, hipStream_t stream) {\n"
+    << "  static int per_cu = physis_hip::BlocksPerCU(\n"
+       "      (const void *)__PSStencilRun2_" << n << "<16>, 64 * 16);\n"
+    << "  const physis_hip::RunShape sh = physis_hip::PlanRun(s.dom, 256, 12, per_cu);\n"
+    << "  if (sh.blocks == 0) return;\n"
+    << "  hipLaunchKernelGGL((__PSStencilRun2_" << n
+    << "<16>), dim3(sh.blocks), dim3(64 * 16), 0, stream, " << Join(largs, ", ")
+    << ");\n}\n\n";
+  // run-time check
+  o << "// The pair (s0, s1) is " << n << "(a -> b), " << n
+    << "(b -> a) over the whole grid with the\n// same scalars: it may run as "
+       "two-sweep passes.\n";
+  o << "static bool __PSFusable2_" << n << "(const struct __PSStencil_" << n
+    << " &s0, const struct __PSStencil_" << n << " &s1) {\n"
+    << "  static const bool off = getenv(\"PHYSIS_HIP_NO_FUSE\") != NULL;\n"
+    << "  const __PSGrid *a = s0." << I << ", *b = s0." << O << ";\n"
+    << "  if (off || !a || !b || a == b || s1." << I << " != b || s1." << O
+    << " != a) return false;\n"
+    << "  if (a->num_dims != 3 || b->num_dims != 3 || a->dim[0] % 4) return false;\n"
+    << "  for (int d = 0; d < 3; ++d) {\n"
+       "    if (a->dim[d] != b->dim[d]) return false;\n"
+       "    if (s0.dom.local_min[d] != 0 || s0.dom.local_max[d] != a->dim[d]) return false;\n"
+       "    if (s1.dom.local_min[d] != 0 || s1.dom.local_max[d] != a->dim[d]) return false;\n"
+       "  }\n";
+  for (auto &a : sa)
+    o << "  if (__builtin_memcmp(&s0." << a << ", &s1." << a << ", sizeof(s0." << a
+      << "))) return false;\n";
+  o << "  return true;\n}\n\n";
+  return o.str();
+}
+
 int Translator::RegEstimate(const KernelInfo &k, int pd) const {
   // double-buffered centre values, the LDS row segments, the staged planes
   // in flight, addressing
@@ -924,6 +1369,7 @@ std::string Translator::EmitKernel(KernelInfo &k) {
   s += EmitDeviceFunction(k);
   s += k.fast ? EmitFastRunKernel(k) : EmitGenericRunKernel(k);
   s += EmitLauncher(k);
+  if (k.fusable) s += EmitFusedRunKernel(k);
   return s;
 }
 
@@ -946,7 +1392,37 @@ std::string Translator::EmitRunFunction(const RunSite &r) {
         << ".__reduce_out ? __PSBlocks_" << k.name << "(" << si
         << ") : 0, iter, " << i << ", " << r.kernels.size() << ");\n";
   }
-  o << "  for (int i = 0; i < iter; ++i) {\n";
+  const bool fuse = r.kernels.size() == 2 && r.kernels[0] == r.kernels[1] &&
+                    kernels_.at(r.kernels[0]).fusable;
+  if (fuse) {
+    // two-sweep passes: an odd count starts with one plain pair; the passes
+    // ping-pong between a and a scratch grid, ending in a, and the last one
+    // also writes its intermediate sweep to b, so a and b end as the plain
+    // pairs leave them
+    const KernelInfo &k = kernels_.at(r.kernels[0]);
+    const GridParam &gi = k.grids[k.fuse_in], &go = k.grids[k.fuse_out];
+    o << "  int i0 = 0;\n"
+      << "  if (iter >= 2 && __PSFusable2_" << k.name << "(s0, s1)) {\n"
+      << "    if (iter % 2) {\n      __PSRun_" << k.name << "(s0, stream, NULL);\n"
+      << "      __PSRun_" << k.name << "(s1, stream, NULL);\n    }\n"
+      << "    physis_hip::CheckGrid3D(s0." << gi.name << ", " << gi.ps_type << ");\n"
+      << "    physis_hip::CheckGrid3D(s0." << go.name << ", " << go.ps_type << ");\n"
+      << "    const int nf = iter - iter % 2;\n"
+      << "    " << gi.dev_view << " va = *(" << gi.dev_view << " *)s0." << gi.name
+      << "->dev;\n"
+      << "    " << gi.dev_view << " vb = *(" << gi.dev_view << " *)s0." << go.name
+      << "->dev;\n"
+      << "    " << gi.dev_view << " vt = va;\n"
+      << "    vt.p = (" << gi.elem << " *)__PSHipScratch(8, (size_t)s0." << gi.name
+      << "->num_elms * sizeof(" << gi.elem << "));\n"
+      << "    " << gi.dev_view << " src = va;\n"
+      << "    for (int j = 0; j < nf; ++j) {\n"
+      << "      const " << gi.dev_view << " dst = (j % 2 == 0) ? vt : va;\n"
+      << "      __PSLaunch2_" << k.name << "(s0, src, dst, j == nf - 1 ? vb.p : NULL, stream);\n"
+      << "      src = dst;\n    }\n"
+      << "    i0 = iter;\n  }\n";
+  }
+  o << "  for (int i = " << (fuse ? "i0" : "0") << "; i < iter; ++i) {\n";
   for (size_t i = 0; i < r.kernels.size(); ++i) {
     const KernelInfo &k = kernels_.at(r.kernels[i]);
     o << "    __PSRun_" << k.name << "(s" << i << ", stream, "

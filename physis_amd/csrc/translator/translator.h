@@ -66,6 +66,10 @@ struct KernelInfo {
   std::string slow_reason;    // why not
   int hy = 0;                 // common y halo of staged grids
   bool used_plain = false, used_reduce = false;
+  // two-sweep fusion (temporal blocking) of map pairs k(a->b), k(b->a)
+  bool fusable = false;
+  int fuse_in = -1, fuse_out = -1;  // grid indices of the read and emitted grids
+  std::string fuse_why;             // why not, for the analysis comment
   int GridIndex(const std::string &n) const {
     for (size_t i = 0; i < grids.size(); ++i)
       if (grids[i].name == n) return (int)i;
@@ -99,6 +103,7 @@ class Translator {
   void AnalyzeGets(KernelInfo &k, const Stmt &s);
   bool RegularIndex(const KernelInfo &k, const Expr &e, int axis, int *off);
   void Decide(KernelInfo &k);
+  void AnalyzeFusion(KernelInfo &k);
 
   // device code
   void RewriteKernelExpr(const KernelInfo &k, ExprP &e);
@@ -110,6 +115,7 @@ class Translator {
   std::string EmitFastRunKernel(const KernelInfo &k);
   std::string EmitGenericRunKernel(const KernelInfo &k);
   std::string EmitLauncher(const KernelInfo &k);
+  std::string EmitFusedRunKernel(const KernelInfo &k);
   // VGPRs per lane of the run kernel at VX = 4 with pd staged planes in
   // flight (a rough estimate), and the prefetch depth used for k.
   int RegEstimate(const KernelInfo &k, int pd) const;

@@ -5,8 +5,8 @@ Run under the profiler, e.g.
         python3 -m physis_amd.profile_kernels
 It issues, in order: a device-to-device copy of one 512 MiB grid (the
 runtime's plain copy kernel, whose FETCH_SIZE / WRITE_SIZE calibrate the
-counters' units), 10 sweeps of the 7-point clamped and periodic run kernels
-at 512^3, 3 PSReduce SUMs at 512^3, 10 Himeno sweeps at 512x256x256, and
+counters' units), at 512^3 four two-sweep launches of the 7-point clamped
+kernel, five single sweeps of it, 10 periodic sweeps, 3 PSReduce SUMs at 512^3, 10 Himeno sweeps at 512x256x256, and
 the precision tier at 512^3: 10 BF16 inner sweeps and 3 fused FP64
 update + residual passes.
 Then `python3 -m physis_amd.profile_summary OUT_fetch OUT_write` writes the
@@ -39,7 +39,9 @@ def main() -> None:
                                  COPY_BYTES) == 0
     assert hip.hipDeviceSynchronize() == 0
     d7 = Diffusion7()
-    d7.run(g1, g2, 10, DIFFUSION_COEF)
+    d7.run(g1, g2, 8, DIFFUSION_COEF)        # 4 two-sweep launches
+    for _ in range(5):                       # the one-sweep kernel
+        d7.sweep_domain(g1, g2, (0, 512, 0, 512, 0, 512), DIFFUSION_COEF)
     d7.run(g1, g2, 10, DIFFUSION_COEF, periodic=True)
     for _ in range(3):
         ps.PSReduce(ps.PS_SUM, g1)

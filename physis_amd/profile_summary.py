@@ -21,8 +21,12 @@ from collections import defaultdict
 
 COPY_BYTES = 512 * 2 ** 20
 KERNELS = [  # (label, name fragment, algorithmic bytes per launch)
+    ("7-pt clamped two-sweep kernel, 512^3 (2 sweeps per launch)",
+     "__PSStencilRun2_diffusion7_kernel<", 8 * 512 ** 3),
     ("7-pt clamped run kernel, 512^3", "__PSStencilRun_diffusion7_kernel<",
      8 * 512 ** 3),
+    ("7-pt periodic two-sweep kernel, 512^3 (2 sweeps per launch)",
+     "__PSStencilRun2_diffusion7_periodic_kernel<", 8 * 512 ** 3),
     ("7-pt periodic run kernel, 512^3",
      "__PSStencilRun_diffusion7_periodic_kernel<", 8 * 512 ** 3),
     ("19-pt Himeno run kernel, 512x256x256", "__PSStencilRun_jacobi_kernel<",
@@ -66,7 +70,7 @@ def main(fetch_dir, write_dir, kt_dir, out_md) -> None:
         for r in csv.DictReader(open(f)):
             kt[r["Name"]] = (float(r["AverageNs"]), int(r["Calls"]))
     rows = []
-    traffic7 = None
+    traffic = {}
     for label, frag, alg in KERNELS:
         name, f, n = _pick(fetch, frag)
         _, w, _ = _pick(write, frag)
@@ -77,7 +81,9 @@ def main(fetch_dir, write_dir, kt_dir, out_md) -> None:
         avg_ns = cands[0][1][0] if cands else float("nan")
         rows.append((label, fb, wb, alg, avg_ns, n))
         if "diffusion7_kernel<" in frag:
-            traffic7 = fb + wb
+            traffic[name.split("(")[0].replace("void ", "").split("<")[0]] = {
+                "hbm_bytes_per_launch": int(fb + wb),
+                "algorithmic_bytes_per_launch": alg}
     lines = [
         "# HBM traffic of the hot-path kernels (MI355X)",
         "",
@@ -113,9 +119,7 @@ def main(fetch_dir, write_dir, kt_dir, out_md) -> None:
     open(out_md, "w").write("\n".join(lines))
     jpath = os.path.join(os.path.dirname(os.path.abspath(out_md)),
                          "diffusion7_512_traffic.json")
-    json.dump({"kernel": "__PSStencilRun_diffusion7_kernel<4, 16, -1>",
-               "hbm_bytes_per_launch": int(traffic7),
-               "algorithmic_bytes_per_launch": 8 * 512 ** 3,
+    json.dump({"kernels": traffic,
                "fetch_bytes_per_unit": fs, "write_bytes_per_unit": ws,
                "source": os.path.basename(out_md)},
               open(jpath, "w"), indent=1)

@@ -86,6 +86,31 @@ def test_shapes(ps, d7, shape, periodic):
     ps.PSGridFree(g2)
 
 
+@pytest.mark.parametrize("shape", [(36, 26, 22), (260, 30, 20), (8, 8, 8),
+                                   (64, 13, 5)])
+@pytest.mark.parametrize("pairs", [1, 2, 3, 5])
+@pytest.mark.parametrize("periodic", [False, True])
+def test_two_sweep_passes_leave_both_grids(ps, d7, shape, pairs, periodic):
+    """PSStencilRun(m12, m21, pairs) of the 7-point kernels runs as two-sweep
+    passes when x is a multiple of 4 (an odd pair count starts with one plain
+    pair): g1 must hold sweep 2*pairs and g2 sweep 2*pairs - 1, bit-exact,
+    across tile edges in x (260 = 256 + 4), y (not a multiple of the 12-row
+    tile) and z chunks."""
+    from physis_amd.seeded import uniform01
+    coef = (0.11, 0.09, 0.12, 0.08, 0.105, 0.095, 0.39)
+    print(f"two-sweep {shape} pairs={pairs} periodic={periodic}", flush=True)
+    a = uniform01(int(np.prod(shape)), seed=23)
+    g1, g2 = grids(ps, shape, a)
+    d7.run(g1, g2, 2 * pairs, coef, periodic=periodic)
+    o1, o2 = a.copy(), a.copy()
+    lib().oracle_diffusion7_run_omp(o1, o2, *shape, 2 * pairs, int(periodic),
+                                    *coef, THREADS)
+    assert_bits(ps.PSGridCopyout(g1), o1, f"g1 {shape} x{2 * pairs}")
+    assert_bits(ps.PSGridCopyout(g2), o2, f"g2 {shape} x{2 * pairs - 1}")
+    ps.PSGridFree(g1)
+    ps.PSGridFree(g2)
+
+
 @pytest.mark.parametrize("dom", [(1, 31, 1, 31, 1, 31), (5, 17, 0, 32, 3, 4),
                                  (0, 32, 7, 8, 0, 32), (13, 14, 2, 30, 9, 25)])
 @pytest.mark.parametrize("periodic", [False, True])
@@ -133,6 +158,7 @@ def test_config2_512cubed_200_sweeps(ps, d7, periodic):
         assert abs(s64 - total0) <= 1e-5 * total0
         assert m <= a.max()
     assert_bits(ps.PSGridCopyout(g1), o1, "512^3 x200")
+    assert_bits(ps.PSGridCopyout(g2), o2, "512^3 x199")
     ps.PSGridFree(g1)
     ps.PSGridFree(g2)
 
