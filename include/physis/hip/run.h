@@ -108,6 +108,24 @@ inline void CheckGrid3D(const __PSGrid *g, int type) {
 
 // The domain of a map must lie inside every grid the map emits to; the run
 // kernel stores at every domain point.
+// The device view of a runtime grid as a run kernel takes it.  The runtime
+// keeps one form per grid, {int dim[3]; void *p}; the 1-D and 2-D views are
+// {int dim[N]; T *p}, whose pointer sits at another offset, so views are
+// built field by field rather than by reinterpreting that form.
+template <typename View>
+inline View DevView(const __PSGrid *g) {
+  View v;
+  constexpr int nd = (int)(sizeof(v.dim) / sizeof(v.dim[0]));
+  if (!g || !g->dev || g->num_dims != nd) {
+    std::fprintf(stderr, "[physis-hip] stencil map on a %s grid where a "
+                 "%d-D grid is expected\n", g ? "mismatched" : "NULL", nd);
+    std::exit(EXIT_FAILURE);
+  }
+  for (int i = 0; i < nd; ++i) v.dim[i] = g->dev->dim[i];
+  v.p = static_cast<decltype(v.p)>(g->dev->p);
+  return v;
+}
+
 inline void CheckEmitDomain(const __PSDomain &dom, const __PSGrid *g) {
   for (int d = 0; d < g->num_dims; ++d) {
     if (dom.local_max[d] <= dom.local_min[d]) return;  // empty domain

@@ -980,9 +980,12 @@ std::string Translator::EmitGenericRunKernel(const KernelInfo &k) {
   o << "template <int RED>\n__global__ __launch_bounds__(256) void "
        "__PSStencilRunGeneric_" << n << "(\n    " << Join(params, ", ")
     << ") {\n";
-  o << "  const int x = dom.local_min[0] + blockIdx.x * 64 + threadIdx.x % 64;\n";
+  // 1-D maps: 256 points per block; otherwise 64 x 4 (x, y) columns
   if (k.ndims >= 2)
-    o << "  const int y = dom.local_min[1] + blockIdx.y * 4 + threadIdx.x / 64;\n";
+    o << "  const int x = dom.local_min[0] + blockIdx.x * 64 + threadIdx.x % 64;\n"
+         "  const int y = dom.local_min[1] + blockIdx.y * 4 + threadIdx.x / 64;\n";
+  else
+    o << "  const int x = dom.local_min[0] + blockIdx.x * 256 + threadIdx.x;\n";
   if (red)
     o << "  " << RT << " __racc = 0;\n  if constexpr (RED >= 0) __racc = "
          "physis_hip::ReduceOp<" << RT << ", RED < 0 ? 0 : RED>::identity();\n";
@@ -1255,7 +1258,7 @@ std::string Translator::EmitLauncher(const KernelInfo &k) {
   const std::string RT = red ? k.ret_type : "float";
   std::vector<std::string> args = {"s.dom"};
   for (auto &g : k.grids)
-    args.push_back("*(" + g.dev_view + " *)s." + g.name + "->dev");
+    args.push_back("physis_hip::DevView<" + g.dev_view + ">(s." + g.name + ")");
   for (auto &s : k.scalars) args.push_back("s." + s.name);
   std::string A = Join(args, ", ");
   if (k.fast) {
@@ -1330,7 +1333,8 @@ std::string Translator::EmitLauncher(const KernelInfo &k) {
       << " &s) {\n  const int nx = s.dom.local_max[0] - s.dom.local_min[0];\n"
       << "  const int ny = " << (k.ndims >= 2 ? "s.dom.local_max[1] - s.dom.local_min[1]" : "1")
       << ";\n  if (nx <= 0 || ny <= 0) return dim3(0, 0, 1);\n"
-      << "  return dim3((nx + 63) / 64, " << (k.ndims >= 2 ? "(ny + 3) / 4" : "1")
+      << "  return dim3(" << (k.ndims >= 2 ? "(nx + 63) / 64, (ny + 3) / 4"
+                                          : "(nx + 255) / 256, 1")
       << ", 1);\n}\n\n";
     o << "static int __PSBlocks_" << n << "(const struct __PSStencil_" << n
       << " &s) {\n  const dim3 g = __PSGridDim_" << n
@@ -1408,10 +1412,10 @@ std::string Translator::EmitRunFunction(const RunSite &r) {
       << "    physis_hip::CheckGrid3D(s0." << gi.name << ", " << gi.ps_type << ");\n"
       << "    physis_hip::CheckGrid3D(s0." << go.name << ", " << go.ps_type << ");\n"
       << "    const int nf = iter - iter % 2;\n"
-      << "    " << gi.dev_view << " va = *(" << gi.dev_view << " *)s0." << gi.name
-      << "->dev;\n"
-      << "    " << gi.dev_view << " vb = *(" << gi.dev_view << " *)s0." << go.name
-      << "->dev;\n"
+      << "    " << gi.dev_view << " va = physis_hip::DevView<" << gi.dev_view
+      << ">(s0." << gi.name << ");\n"
+      << "    " << gi.dev_view << " vb = physis_hip::DevView<" << gi.dev_view
+      << ">(s0." << go.name << ");\n"
       << "    " << gi.dev_view << " vt = va;\n"
       << "    vt.p = (" << gi.elem << " *)__PSHipScratch(8, (size_t)s0." << gi.name
       << "->num_elms * sizeof(" << gi.elem << "));\n"

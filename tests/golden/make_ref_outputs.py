@@ -19,10 +19,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 REF = os.path.join(ROOT, "oracle", "_ref")
 
-SYSTEM_TESTS = ["test_7-pt", "test_7-pt-periodic", "test_7-pt-neumann-cond",
-                "test_7-pt-multi-iterations", "test_asymmetric",
-                "test_asymmetric-periodic", "test_27-pt", "test_27-pt-periodic",
-                "test_27-pt-reduction", "test_reduction-3d-sum"]
+TEST_CASES = "/root/reference/tests/system_tests/test_cases"
+# Every hand-written REFERENCE-target oracle of the system tests.
+SYSTEM_TESTS = sorted(f[:-len(".manual.ref.c")] for f in os.listdir(TEST_CASES)
+                      if f.endswith(".manual.ref.c"))
 # (size, count) of the C++ diffusion baseline runs.
 DIFFUSION_RUNS = [(32, 4), (24, 10)]
 # (nx, ny, nz, sweeps, seed) of the original Himeno runs.

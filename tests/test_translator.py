@@ -87,3 +87,27 @@ def test_missing_kernel_is_an_error():
     src = PROG.replace("PSStencilMap(k,", "PSStencilMap(nokernel,") % "z"
     rc, _, err = translate(None, src)
     assert rc != 0 and "nokernel" in err
+
+
+REF_CASES = "/root/reference/tests/system_tests/test_cases"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_CASES),
+                    reason="the reference's system tests are not present")
+def test_reference_corpus_translates():
+    """Every program of the reference's system-test corpus (unchanged) is
+    translated, except the features named in the error of the rest."""
+    from test_gpu_refcorpus import CORPUS, UNSUPPORTED
+    inc = os.path.join(ROOT, "include")
+    for name in CORPUS:
+        src = os.path.join(REF_CASES, name + ".c")
+        with tempfile.TemporaryDirectory() as d:
+            p = subprocess.run([PHYSISC, "-I", inc, src, "-o",
+                                os.path.join(d, "o.hip.cpp")],
+                               capture_output=True, text=True)
+        if name in UNSUPPORTED:
+            assert p.returncode != 0, name
+            assert "red-black" in p.stderr or "DeclareGrid3D" in p.stderr or \
+                "user-defined" in p.stderr, (name, p.stderr)
+        else:
+            assert p.returncode == 0, (name, p.stderr)
