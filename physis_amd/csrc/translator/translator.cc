@@ -97,6 +97,19 @@ std::string Join(const std::vector<std::string> &v, const std::string &sep) {
   return s;
 }
 
+// PSStencilMap and its red-black forms (physis_user.h:98-105); the colour
+// argument of the run kernel: -1 all points, 0 red, 1 black, 2 red then
+// black.
+bool IsMapName(const std::string &c, int *rb = nullptr) {
+  int v = -2;
+  if (c == "PSStencilMap") v = -1;
+  else if (c == "PSStencilMapRed") v = 0;
+  else if (c == "PSStencilMapBlack") v = 1;
+  else if (c == "PSStencilMapRedBlack") v = 2;
+  if (rb) *rb = v;
+  return v != -2;
+}
+
 }  // namespace
 
 Translator::Translator(
@@ -116,7 +129,8 @@ void Translator::FindKernelsIn(const Expr &e) {
   if (const std::string *c = CalleeName(e)) {
     int karg = -1;
     bool reduce = false;
-    if (*c == "PSStencilMap" && e.kids.size() >= 3) karg = 1;
+    int rb = -2;
+    if (IsMapName(*c, &rb) && e.kids.size() >= 3) karg = 1;
     if (*c == "PSStencilMapReduce" && e.kids.size() >= 5) {
       karg = 3;
       reduce = true;
@@ -125,10 +139,6 @@ void Translator::FindKernelsIn(const Expr &e) {
       karg = 3;
       reduce = true;
     }
-    if (*c == "PSStencilMapRedBlack" || *c == "PSStencilMapRed" ||
-        *c == "PSStencilMapBlack")
-      Fail(prog_.file, "red-black stencil maps are not supported by the HIP "
-                       "target yet");
     if (karg > 0) {
       const Expr *k = e.kids[karg].get();
       while (k->kind == EK::kParen || k->kind == EK::kCast) k = k->kids[0].get();
@@ -147,6 +157,7 @@ void Translator::FindKernelsIn(const Expr &e) {
         kernel_order_.push_back(name);
       }
       (reduce ? kernels_[name].used_reduce : kernels_[name].used_plain) = true;
+      if (rb >= 0) kernels_[name].used_rb = true;
     }
   }
   for (const auto &k : e.kids)
@@ -538,6 +549,29 @@ void Translator::AnalyzeFusion(KernelInfo &k) {
 void Translator::Decide(KernelInfo &k) {
   k.fast = k.ndims == 3;
   if (!k.fast) k.slow_reason = "not a 3-D stencil";
+  if (k.used_rb) {
+    k.fast = false;
+    k.slow_reason = "red-black map";
+    // A colour sweep updates its grid in place.  REF visits the points of
+    // a colour in loop order, so it equals a parallel sweep only when no
+    // point reads another point of its own colour: every regular get of an
+    // emitted grid other than the centre is at an odd offset sum, and none
+    // is irregular.
+    bool exact = true;
+    for (auto &g : k.grids) {
+      if (!g.emitted || !g.read) continue;
+      if (g.irregular) exact = false;
+      for (auto &o : g.offsets)
+        if ((o[0] || o[1] || o[2]) && ((o[0] + o[1] + o[2]) % 2 == 0))
+          exact = false;
+    }
+    k.rb_note = exact ? "in-place colour sweep reads only the other colour"
+                      : "in-place colour sweep reads its own colour: results "
+                        "depend on the order of updates";
+    if (!exact)
+      std::fprintf(stderr, "physisc-hip: warning: %s: %s\n", k.name.c_str(),
+                   k.rb_note.c_str());
+  }
   for (auto &g : k.grids)
     if (g.dims != 3 && (g.read || g.emitted)) {
       k.fast = false;
@@ -652,6 +686,8 @@ std::string Translator::EmitStencilStruct(const KernelInfo &k) {
   if (k.ret_type != "void")
     o << "  " << k.ret_type << " *__reduce_out;  // PSStencilMapReduce target\n"
       << "  int __reduce_op;\n";
+  if (k.used_rb)
+    o << "  int __rb;  // -1 all points, 0 red, 1 black, 2 red then black\n";
   o << "};\n\n";
   std::vector<std::string> params = {"__PSDomain dom"}, init = {"dom"};
   for (auto &p : k.order) {
@@ -670,9 +706,20 @@ std::string Translator::EmitStencilStruct(const KernelInfo &k) {
     init.push_back("NULL");
     init.push_back("0");
   }
+  if (k.used_rb) init.push_back("-1");
   o << "static struct __PSStencil_" << n << " __PSStencilMap_" << n << "(\n    "
     << Join(params, ", ") << ") {\n  struct __PSStencil_" << n << " s = {"
     << Join(init, ", ") << "};\n  return s;\n}\n\n";
+  if (k.used_rb) {
+    std::vector<std::string> args;
+    for (auto &pp : params) args.push_back(pp.substr(pp.find_last_of(" *") + 1));
+    const char *names[] = {"Red", "Black", "RedBlack"};
+    for (int c = 0; c < 3; ++c)
+      o << "static struct __PSStencil_" << n << " __PSStencilMap" << names[c]
+        << "_" << n << "(\n    " << Join(params, ", ") << ") {\n  struct __PSStencil_"
+        << n << " s = __PSStencilMap_" << n << "(" << Join(args, ", ")
+        << ");\n  s.__rb = " << c << ";\n  return s;\n}\n\n";
+  }
   if (k.ret_type != "void") {
     std::vector<std::string> args;
     for (auto &pp : params) args.push_back(pp.substr(pp.find_last_of(" *") + 1));
@@ -714,6 +761,7 @@ std::string Translator::EmitContext(const KernelInfo &k) {
       o << "\n";
     }
   }
+  if (k.used_rb) o << "//   red-black: " << k.rb_note << "\n";
   if (k.fast)
     o << "//   two-sweep fusion of map pairs: "
       << (k.fusable ? std::string("yes") : "no (" + k.fuse_why + ")") << "\n";
@@ -975,6 +1023,7 @@ std::string Translator::EmitGenericRunKernel(const KernelInfo &k) {
   for (auto &g : k.grids) params.push_back(g.dev_view + " " + g.name);
   for (auto &s : k.scalars) params.push_back(s.type + " " + s.name);
   params.push_back(RT + " *__restrict__ __red");
+  if (k.used_rb) params.push_back("int __rb");
   o << "// Generic run kernel (one thread per (x, y) column marching z, all\n"
        "// gets read from memory): " << k.slow_reason << ".\n";
   o << "template <int RED>\n__global__ __launch_bounds__(256) void "
@@ -1001,7 +1050,16 @@ std::string Translator::EmitGenericRunKernel(const KernelInfo &k) {
   if (k.ndims == 3)
     o << "    for (int z = dom.local_min[2]; z < dom.local_max[2]; ++z) {\n";
   else
-    o << "    {\n";
+    o << "    do {\n";
+  if (k.used_rb) {
+    // the REF colour loop (reference_runtime_builder.cc:585-599, 634-646)
+    // starts x at min + ((min & 1) ^ ((rb + y + z) % 2)) in steps of 2,
+    // i.e. visits the points with x + y + z + rb even
+    std::string sum = "x";
+    if (k.ndims >= 2) sum += " + y";
+    if (k.ndims >= 3) sum += " + z";
+    o << "      if (__rb >= 0 && ((" << sum << " + __rb) & 1)) continue;\n";
+  }
   for (auto &g : k.grids)
     if (g.emitted) o << "      ctx." << g.name << "_e = false;\n";
   std::vector<std::string> args = {"x"};
@@ -1024,7 +1082,7 @@ std::string Translator::EmitGenericRunKernel(const KernelInfo &k) {
     o << "      if (ctx." << g.name << "_e) " << g.name << ".p[" << off
       << "] = ctx." << g.name << "_v;\n";
   }
-  o << "    }\n  }\n";
+  o << (k.ndims == 3 ? "    }\n  }\n" : "    } while (0);\n  }\n");
   if (red)
     o << "  if constexpr (RED >= 0) {\n    __shared__ " << RT
       << " __lds_red[4];\n    const " << RT << " __r = physis_hip::BlockReduce<"
@@ -1348,19 +1406,26 @@ std::string Translator::EmitLauncher(const KernelInfo &k) {
       o << "  if (s.dom.local_max[2] <= s.dom.local_min[2]) return 0;\n";
     o << "  const dim3 grid = __PSGridDim_" << n << "(s);\n"
       << "  if (grid.x == 0) return 0;\n";
+    // red-black maps: one launch per colour, red first
+    const std::string RBA = k.used_rb ? ", __c" : "";
+    if (k.used_rb)
+      o << "  for (int __c = s.__rb == 2 ? 0 : s.__rb; "
+           "__c <= (s.__rb == 2 ? 1 : s.__rb); ++__c) {\n";
+    else
+      o << "  {\n";
     if (red && k.used_reduce) {
       o << "  switch (red ? s.__reduce_op : -1) {\n";
       for (const char *op : {"PS_MAX", "PS_MIN", "PS_SUM", "PS_PROD"})
         o << "    case " << op << ":\n      hipLaunchKernelGGL((__PSStencilRunGeneric_"
           << n << "<" << op << ">), grid, dim3(256), 0, stream, " << A << ", ("
-          << RT << " *)red);\n      break;\n";
+          << RT << " *)red" << RBA << ");\n      break;\n";
       o << "    default:\n";
     } else {
       o << "  (void)red;\n  {\n";
     }
     o << "      hipLaunchKernelGGL((__PSStencilRunGeneric_" << n
       << "<-1>), grid, dim3(256), 0, stream, " << A << ", (" << RT
-      << " *)NULL);\n  }\n  return grid.x * grid.y;\n}\n\n";
+      << " *)NULL" << RBA << ");\n  }\n  }\n  return grid.x * grid.y;\n}\n\n";
   }
   return o.str();
 }
@@ -1463,11 +1528,11 @@ ExprP Translator::MapCall(Expr &call, const Function &f) {
   (void)f;
   auto out = std::make_unique<Expr>(EK::kCall);
   auto callee = std::make_unique<Expr>(EK::kIdent);
-  if (c == "PSStencilMap") {
-    std::string k = call.kids[1]->text;
+  if (IsMapName(c)) {
     while (call.kids[1]->kind == EK::kParen || call.kids[1]->kind == EK::kCast)
       call.kids[1] = std::move(call.kids[1]->kids[0]);
-    callee->text = "__PSStencilMap_" + call.kids[1]->text;
+    // __PSStencilMap_k, __PSStencilMapRed_k, ...
+    callee->text = "__" + c + "_" + call.kids[1]->text;
     out->kids.push_back(std::move(callee));
     for (size_t i = 2; i < call.kids.size(); ++i)
       out->kids.push_back(std::move(call.kids[i]));
@@ -1497,8 +1562,8 @@ void Translator::RewriteHostExpr(ExprP &e, const Function &f) {
     for (size_t i = 1; i < e->kids.size(); ++i) {
       Expr &a = *e->kids[i];
       const std::string *ac = CalleeName(a);
-      if (ac && (*ac == "PSStencilMap" || *ac == "PSStencilMapReduce")) {
-        const Expr *kn = (*ac == "PSStencilMap" ? a.kids[1] : a.kids[3]).get();
+      if (ac && (IsMapName(*ac) || *ac == "PSStencilMapReduce")) {
+        const Expr *kn = (IsMapName(*ac) ? a.kids[1] : a.kids[3]).get();
         while (kn->kind == EK::kParen || kn->kind == EK::kCast) kn = kn->kids[0].get();
         site.kernels.push_back(kn->text);
         for (size_t j = 2; j < a.kids.size(); ++j) RewriteHostExpr(a.kids[j], f);

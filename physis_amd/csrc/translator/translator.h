@@ -66,6 +66,10 @@ struct KernelInfo {
   std::string slow_reason;    // why not
   int hy = 0;                 // common y halo of staged grids
   bool used_plain = false, used_reduce = false;
+  // used in a red-black map (PSStencilMapRed/Black/RedBlack): the run
+  // kernel visits one colour, x + y + z + rb even (rb 0 red, 1 black)
+  bool used_rb = false;
+  std::string rb_note;  // whether the in-place colour sweep matches REF order
   // two-sweep fusion (temporal blocking) of map pairs k(a->b), k(b->a)
   bool fusable = false;
   int fuse_in = -1, fuse_out = -1;  // grid indices of the read and emitted grids

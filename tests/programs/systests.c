@@ -238,3 +238,164 @@ void systest_reduction_3d_sum(float *v) {
   PSGridFree(g1);
   free(indata);
 }
+
+/* ---- red-black, 1-D, 2-D, mixed-dimension and hole stencils */
+
+/* test_redblack.c:13-21: the 7-point sum written back in place. */
+static void krb(const int x, const int y, const int z, PSGrid3DFloat g) {
+  float v = PSGridGet(g, x, y, z) +
+      PSGridGet(g, x+1, y, z) + PSGridGet(g, x-1, y, z) +
+      PSGridGet(g, x, y+1, z) + PSGridGet(g, x, y-1, z) +
+      PSGridGet(g, x, y, z-1) + PSGridGet(g, x, y, z+1);
+  PSGridEmit(g, v);
+  return;
+}
+
+/* test_redblack-periodic.c:13-21 */
+static void krbp(const int x, const int y, const int z, PSGrid3DFloat g) {
+  float v = PSGridGetPeriodic(g, x, y, z) +
+      PSGridGetPeriodic(g, x+1, y, z) + PSGridGetPeriodic(g, x-1, y, z) +
+      PSGridGetPeriodic(g, x, y+1, z) + PSGridGetPeriodic(g, x, y-1, z) +
+      PSGridGetPeriodic(g, x, y, z-1) + PSGridGetPeriodic(g, x, y, z+1);
+  PSGridEmit(g, v);
+  return;
+}
+
+/* One grid of N^3 initialised to i, swept by a red-black map (mode 0: both
+ * colours in one map; 1: red and black as two maps of one run; 2: periodic
+ * over the whole grid). */
+static void run_redblack(int mode, float *out) {
+  PSGrid3DFloat g = PSGrid3DFloatNew(N, N, N);
+  PSDomain3D d = PSDomain3DNew(1, N - 1, 1, N - 1, 1, N - 1);
+  PSDomain3D dall = PSDomain3DNew(0, N, 0, N, 0, N);
+  float *indata = (float *)malloc(sizeof(float) * N * N * N);
+  int i;
+  for (i = 0; i < N * N * N; i++) indata[i] = i;
+  PSGridCopyin(g, indata);
+  if (mode == 0) PSStencilRun(PSStencilMapRedBlack(krb, d, g));
+  if (mode == 1)
+    PSStencilRun(PSStencilMapRed(krb, d, g), PSStencilMapBlack(krb, d, g));
+  if (mode == 2) PSStencilRun(PSStencilMapRedBlack(krbp, dall, g));
+  PSGridCopyout(g, out);
+  PSGridFree(g);
+  free(indata);
+}
+
+void systest_redblack(float *out) { run_redblack(0, out); }
+void systest_redblack_separated(float *out) { run_redblack(1, out); }
+void systest_redblack_periodic(float *out) { run_redblack(2, out); }
+
+/* test_3-pt-1d.c:12-19 */
+static void k3(const int x, PSGrid1DFloat g, PSGrid1DFloat g2) {
+  float v = PSGridGet(g, x-1) + PSGridGet(g, x) + PSGridGet(g, x+1);
+  PSGridEmit(g2, v);
+  return;
+}
+
+/* test_5-pt-2d.c:12-21 */
+static void k5(const int x, const int y, PSGrid2DFloat g, PSGrid2DFloat g2) {
+  float v = PSGridGet(g, x, y) + PSGridGet(g, x-1, y) + PSGridGet(g, x+1, y) +
+      PSGridGet(g, x, y-1) + PSGridGet(g, x, y+1);
+  PSGridEmit(g2, v);
+  return;
+}
+
+/* test_3-pt-1d.c main: N = 1024, domain [1, N-1). */
+void systest_3pt_1d(float *out) {
+  int n = 1024, i;
+  PSGrid1DFloat g1 = PSGrid1DFloatNew(n);
+  PSGrid1DFloat g2 = PSGrid1DFloatNew(n);
+  PSDomain1D d = PSDomain1DNew(1, n - 1);
+  float *indata = (float *)malloc(sizeof(float) * n);
+  for (i = 0; i < n; i++) indata[i] = i;
+  PSGridCopyin(g1, indata);
+  PSGridCopyin(g2, indata);
+  PSStencilRun(PSStencilMap(k3, d, g1, g2));
+  PSGridCopyout(g2, out);
+  PSGridFree(g1);
+  PSGridFree(g2);
+  free(indata);
+}
+
+/* test_5-pt-2d.c main: N x N, domain [1, N-1)^2. */
+void systest_5pt_2d(float *out) {
+  int i;
+  PSGrid2DFloat g1 = PSGrid2DFloatNew(N, N);
+  PSGrid2DFloat g2 = PSGrid2DFloatNew(N, N);
+  PSDomain2D d = PSDomain2DNew(1, N - 1, 1, N - 1);
+  float *indata = (float *)malloc(sizeof(float) * N * N);
+  for (i = 0; i < N * N; i++) indata[i] = i;
+  PSGridCopyin(g1, indata);
+  PSGridCopyin(g2, indata);
+  PSStencilRun(PSStencilMap(k5, d, g1, g2));
+  PSGridCopyout(g2, out);
+  PSGridFree(g1);
+  PSGridFree(g2);
+  free(indata);
+}
+
+/* test_mixed-dim3.c:10-20: a 3-D stencil weighted by a 1-D grid along x and
+ * a 2-D grid over (y, z). */
+static void kmix(const int x, const int y, const int z, PSGrid3DFloat g,
+                 PSGrid3DFloat g2, PSGrid1DFloat i, PSGrid2DFloat j) {
+  float v = PSGridGet(g, x, y, z) +
+      PSGridGet(g, x-1, y, z) * PSGridGet(i, x-1) +
+      PSGridGet(g, x+1, y, z) * PSGridGet(i, x+1) +
+      PSGridGet(g, x, y-1, z) * PSGridGet(j, y-1, z) +
+      PSGridGet(g, x, y+1, z) * PSGridGet(j, y+1, z) +
+      PSGridGet(g, x, y, z-1) * PSGridGet(j, y, z-1) +
+      PSGridGet(g, x, y, z+1) * PSGridGet(j, y, z+1);
+  PSGridEmit(g2, v);
+  return;
+}
+
+/* test_mixed-dim3.c main: grids N x (N+2) x N, weights 1 or 2 alternating. */
+void systest_mixed_dim3(float *out) {
+  int ny = N + 2, i;
+  size_t nelms = (size_t)N * ny * N;
+  PSGrid3DFloat g1 = PSGrid3DFloatNew(N, ny, N);
+  PSGrid3DFloat g2 = PSGrid3DFloatNew(N, ny, N);
+  PSGrid1DFloat cx = PSGrid1DFloatNew(N);
+  PSGrid2DFloat cy = PSGrid2DFloatNew(ny, N);
+  PSDomain3D d = PSDomain3DNew(1, N - 1, 1, ny - 1, 1, N - 1);
+  float *indata = (float *)malloc(sizeof(float) * nelms);
+  for (i = 0; i < (int)nelms; i++) indata[i] = i;
+  PSGridCopyin(g1, indata);
+  PSGridCopyin(g2, indata);
+  for (i = 0; i < N; i++) indata[i] = 1 + (i % 2);
+  PSGridCopyin(cx, indata);
+  for (i = 0; i < ny * N; i++) indata[i] = 1 + (i % 2);
+  PSGridCopyin(cy, indata);
+  PSStencilRun(PSStencilMap(kmix, d, g1, g2, cx, cy));
+  PSGridCopyout(g2, out);
+  PSGridFree(g1);
+  PSGridFree(g2);
+  PSGridFree(cx);
+  PSGridFree(cy);
+  free(indata);
+}
+
+/* test_stencil-hole.c:12-18: x offsets of +-2 with no +-1 in between. */
+static void khole(const int x, const int y, const int z, PSGrid3DFloat g,
+                  PSGrid3DFloat g2) {
+  float v = PSGridGet(g, x, y, z) +
+      PSGridGet(g, x+2, y, z) + PSGridGet(g, x-2, y, z);
+  PSGridEmit(g2, v);
+  return;
+}
+
+void systest_stencil_hole(float *out) {
+  int i;
+  PSGrid3DFloat g1 = PSGrid3DFloatNew(N, N, N);
+  PSGrid3DFloat g2 = PSGrid3DFloatNew(N, N, N);
+  PSDomain3D d = PSDomain3DNew(2, N - 2, 2, N - 2, 2, N - 2);
+  float *indata = (float *)malloc(sizeof(float) * N * N * N);
+  for (i = 0; i < N * N * N; i++) indata[i] = i;
+  PSGridCopyin(g1, indata);
+  PSGridCopyin(g2, indata);
+  PSStencilRun(PSStencilMap(khole, d, g1, g2));
+  PSGridCopyout(g2, out);
+  PSGridFree(g1);
+  PSGridFree(g2);
+  free(indata);
+}
