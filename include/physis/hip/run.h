@@ -25,6 +25,13 @@ inline __PSGrid *NewGrid(int type, int size, int num_dims, int d0, int d1 = 1,
   return __PSGridNew(&ti, num_dims, dim, nullptr);
 }
 
+// PSGrid<n>D<Name>New for a user-defined point type: ti describes its members.
+inline __PSGrid *NewUserGrid(__PSGridTypeInfo *ti, int num_dims, int d0,
+                             int d1 = 1, int d2 = 1) {
+  PSVectorInt dim = {d0, d1, d2};
+  return __PSGridNew(ti, num_dims, dim, nullptr);
+}
+
 // PSGridSet(g, i..., v) with v converted to the element type.
 template <typename T>
 inline void SetElem(__PSGrid *g, T v, PSIndex i0, PSIndex i1 = 0,

@@ -364,6 +364,37 @@ __device__ __forceinline__ T GetLinear(const T *p, const int *dim, int nd,
   if (nd > 2) { off += (long)i2 * dim[0] * dim[1]; n *= dim[2]; }
   return (off >= 0 && off < n) ? p[off] : T(0);
 }
+// Offsets of gets on grids of user-defined points (one plane per scalar
+// component): the linear offset of the index, or -1 where GetLinear would
+// read 0; the wrapped offset of a periodic get.
+__device__ __forceinline__ long OffsetLinear(const int *dim, int nd, int i0,
+                                             int i1 = 0, int i2 = 0) {
+  long off = i0;
+  long n = dim[0];
+  if (nd > 1) { off += (long)i1 * dim[0]; n *= dim[1]; }
+  if (nd > 2) { off += (long)i2 * dim[0] * dim[1]; n *= dim[2]; }
+  return (off >= 0 && off < n) ? off : -1;
+}
+__device__ __forceinline__ long OffsetPeriodic(const int *dim, int nd, int i0,
+                                               int i1 = 0, int i2 = 0) {
+  long off = WrapIndex(i0, dim[0]);
+  if (nd > 1) off += (long)WrapIndex(i1, dim[1]) * dim[0];
+  if (nd > 2) off += (long)WrapIndex(i2, dim[2]) * dim[0] * dim[1];
+  return off;
+}
+// Component `comp` (of ncomp) of a member at point offset `off` (n points
+// per plane): 0 outside; stores outside are dropped.
+template <typename T>
+__device__ __forceinline__ T UdtLoad(const T *p, long n, int ncomp, long comp,
+                                     long off) {
+  return (off >= 0 && comp >= 0 && comp < ncomp) ? p[comp * n + off] : T(0);
+}
+template <typename T>
+__device__ __forceinline__ void UdtStore(T *p, long n, int ncomp, long comp,
+                                         long off, T v) {
+  if (off >= 0 && off < n && comp >= 0 && comp < ncomp) p[comp * n + off] = v;
+}
+
 template <typename T>
 __device__ __forceinline__ T GetPeriodic(const T *p, const int *dim, int nd,
                                          int i0, int i1 = 0, int i2 = 0) {

@@ -37,6 +37,7 @@ RUNTIME_SOURCES = [
     os.path.join(CSRC, "runtime", "libphysis_rt_hip.cc"),
     os.path.join(CSRC, "runtime", "reduce_hip.hip"),
     os.path.join(CSRC, "runtime", "buffer_hip.hip"),
+    os.path.join(CSRC, "runtime", "grid_user_hip.hip"),
 ]
 
 # Poisson solver extensions (precision tier, plain FP32 solve) over the

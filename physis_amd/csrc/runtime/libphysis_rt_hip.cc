@@ -86,6 +86,45 @@ PSType TypeFromSize(const __PSGridTypeInfo *ti) {
   return PS_FLOAT;
 }
 
+// The member layout of a grid of user-defined points.  Host offsets follow
+// the C rules for a struct of scalars and arrays of scalars (each member at
+// the next multiple of its scalar size, the struct padded to its largest
+// one); a type info that does not add up to the struct's size is rejected.
+UserLayout *NewUserLayout(const __PSGridTypeInfo *ti, int64_t num_elms) {
+  if (ti->num_members > UserLayout::kMaxMembers)
+    PS_FATAL("point type with %d members (at most %d)", ti->num_members,
+             UserLayout::kMaxMembers);
+  UserLayout *l = new UserLayout;
+  l->num_members = ti->num_members;
+  size_t host = 0, dev = 0;
+  int align = 1;
+  for (int m = 0; m < ti->num_members; ++m) {
+    const __PSGridTypeMemberInfo &mi = ti->members[m];
+    const int sz = mi.size;
+    if (sz != 1 && sz != 2 && sz != 4 && sz != 8)
+      PS_FATAL("member %d: scalars of %d bytes are not supported", m, sz);
+    if (mi.rank < 0 || mi.rank > PS_GRID_USER_TYPE_MAX_ARRAY_RANK)
+      PS_FATAL("member %d: array rank %d", m, mi.rank);
+    int comps = 1;
+    for (int r = 0; r < mi.rank; ++r) comps *= mi.dim[r];
+    l->size[m] = sz;
+    l->comps[m] = comps;
+    host = (host + sz - 1) / sz * sz;
+    l->host_off[m] = host;
+    host += (size_t)sz * comps;
+    align = sz > align ? sz : align;
+    dev = (dev + 255) / 256 * 256;
+    l->dev_off[m] = dev;
+    dev += (size_t)sz * comps * (size_t)num_elms;
+  }
+  host = (host + align - 1) / align * align;
+  if ((int64_t)host != ti->size)
+    PS_FATAL("point type of %d bytes whose members add up to %zu", ti->size,
+             host);
+  l->bytes = dev;
+  return l;
+}
+
 void CopyIndices(va_list vl, int nd, PSIndex *idx) {
   for (int i = 0; i < nd; ++i) idx[i] = va_arg(vl, PSIndex);
 }
@@ -156,6 +195,12 @@ __PSGrid *__PSGridNew(__PSGridTypeInfo *type_info, int num_dims,
     return g;
   }
   size_t bytes = (size_t)g->num_elms * (size_t)g->elm_size;
+  if (type_info && type_info->num_members > 0) {
+    UserLayout *l = NewUserLayout(type_info, g->num_elms);
+    g->user = l;
+    g->type = PS_USER;
+    bytes = l->bytes;
+  }
   void *buf = nullptr;
   if (bytes > 0) {
     hipError_t e = hipMalloc(&buf, bytes);
@@ -164,6 +209,7 @@ __PSGrid *__PSGridNew(__PSGridTypeInfo *type_info, int num_dims,
       std::fprintf(stderr,
                    "[physis-hip] __PSGridNew: cannot allocate %zu bytes: %s\n",
                    bytes, hipGetErrorString(e));
+      delete (UserLayout *)g->user;
       std::free(g);
       return INVALID_GRID;
     }
@@ -192,6 +238,8 @@ void __PSGridFree(void *gv, __PSGrid_devFreeFunc func) {
     }
     g->dev = nullptr;
   }
+  delete (UserLayout *)g->user;
+  g->user = nullptr;
   g->p = nullptr;
   std::free(g);
 }
@@ -205,6 +253,16 @@ void __PSGridCopyin(void *gv, const void *src_array,
   }
   size_t bytes = (size_t)g->num_elms * (size_t)g->elm_size;
   if (bytes == 0) return;
+  if (g->user) {
+    // stage the host structs on the device, then split them into planes
+    void *stage = Scratch(0, bytes);
+    PS_HIP_CHECK(hipMemcpyAsync(stage, src_array, bytes, hipMemcpyHostToDevice,
+                                rt().stream));
+    UserAosToSoa(*(const UserLayout *)g->user, stage, (size_t)g->elm_size,
+                 g->p, g->num_elms, rt().stream);
+    PS_HIP_CHECK(hipStreamSynchronize(rt().stream));
+    return;
+  }
   PS_HIP_CHECK(hipMemcpyAsync(g->p, src_array, bytes, hipMemcpyHostToDevice,
                               rt().stream));
   PS_HIP_CHECK(hipStreamSynchronize(rt().stream));
@@ -219,9 +277,31 @@ void __PSGridCopyout(void *gv, void *dst_array,
   }
   size_t bytes = (size_t)g->num_elms * (size_t)g->elm_size;
   if (bytes == 0) return;
+  if (g->user) {
+    // gather the planes into structs on the device (padding bytes zero),
+    // then copy them out
+    void *stage = Scratch(0, bytes);
+    PS_HIP_CHECK(hipMemsetAsync(stage, 0, bytes, rt().stream));
+    UserSoaToAos(*(const UserLayout *)g->user, g->p, stage,
+                 (size_t)g->elm_size, g->num_elms, rt().stream);
+    PS_HIP_CHECK(hipMemcpyAsync(dst_array, stage, bytes, hipMemcpyDeviceToHost,
+                                rt().stream));
+    PS_HIP_CHECK(hipStreamSynchronize(rt().stream));
+    return;
+  }
   PS_HIP_CHECK(hipMemcpyAsync(dst_array, g->p, bytes, hipMemcpyDeviceToHost,
                               rt().stream));
   PS_HIP_CHECK(hipStreamSynchronize(rt().stream));
+}
+
+void *__PSGridUserMember(__PSGrid *g, int member) {
+  if (!g || !g->user)
+    PS_FATAL("__PSGridUserMember on a grid of a primitive type");
+  const UserLayout *l = (const UserLayout *)g->user;
+  if (member < 0 || member >= l->num_members)
+    PS_FATAL("member %d of a point type with %d members", member,
+             l->num_members);
+  return (char *)g->p + l->dev_off[member];
 }
 
 void __PSGridSwap(__PSGrid *g) { (void)g; }
@@ -232,6 +312,7 @@ void __PSGridSet(__PSGrid *g, void *buf, ...) {
   va_start(vl, buf);
   CopyIndices(vl, g->num_dims, idx);
   va_end(vl);
+  if (g->user) PS_FATAL("PSGridSet on a grid of user-defined points");
   int64_t off = LinearOffset(g, idx) * g->elm_size;
   PS_HIP_CHECK(hipMemcpyAsync((char *)g->p + off, buf, g->elm_size,
                               hipMemcpyHostToDevice, rt().stream));
@@ -244,6 +325,7 @@ void __PSGridGet(__PSGrid *g, void *buf, ...) {
   va_start(vl, buf);
   CopyIndices(vl, g->num_dims, idx);
   va_end(vl);
+  if (g->user) PS_FATAL("PSGridGet on a grid of user-defined points");
   int64_t off = LinearOffset(g, idx) * g->elm_size;
   PS_HIP_CHECK(hipMemcpyAsync(buf, (const char *)g->p + off, g->elm_size,
                               hipMemcpyDeviceToHost, rt().stream));

@@ -149,6 +149,7 @@ void ReduceGrid(void *buf, PSReduceOp op, const __PSGrid *g, PSType type) {
     case PS_LONG: want_size = sizeof(long); break;
     default: PS_FATAL("PSReduce on unsupported element type %d", (int)type);
   }
+  if (g->user) PS_FATAL("PSReduce on a grid of user-defined points");
   if (g->elm_size != want_size)
     PS_FATAL("PSReduce: grid element size %d does not match the %d-byte "
              "reduction type", g->elm_size, want_size);

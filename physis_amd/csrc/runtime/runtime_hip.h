@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 
@@ -52,6 +53,29 @@ void *HostResultSlot();
 
 // reduce_hip.hip
 void ReduceGrid(void *buf, PSReduceOp op, const __PSGrid *g, PSType type);
+
+// Storage of a grid of user-defined points (type PS_USER with members):
+// member by member, each scalar component of a member (array members have
+// several, row-major) a dense plane of num_elms values in the grid's own
+// index order; member m's first plane starts dev_off[m] bytes into the
+// buffer (256-byte aligned), component c of it c * num_elms * size[m] bytes
+// later.  host_off[m] is the member's byte offset in the C struct the host
+// arrays hold.
+struct UserLayout {
+  static constexpr int kMaxMembers = 64;
+  int num_members = 0;
+  int size[kMaxMembers];        // bytes of one scalar of the member
+  int comps[kMaxMembers];       // scalars per point (product of extents)
+  size_t host_off[kMaxMembers];
+  size_t dev_off[kMaxMembers];
+  size_t bytes = 0;             // device buffer size
+};
+
+// grid_user_hip.hip: host AoS array (staged in device memory) <-> planes
+void UserAosToSoa(const UserLayout &l, const void *aos_dev, size_t stride,
+                  void *soa, int64_t n, hipStream_t s);
+void UserSoaToAos(const UserLayout &l, const void *soa, void *aos_dev,
+                  size_t stride, int64_t n, hipStream_t s);
 
 }  // namespace physis_hip
 

@@ -114,6 +114,7 @@ struct TopItem {
   std::unique_ptr<Function> func;
   std::vector<Token> tokens;   // kTokens: verbatim declaration
   std::string include_line;    // kInclude
+  std::string user_grid;       // kTokens of DeclareGrid<n>D(Name, type): Name
 };
 
 struct Program {

@@ -4,6 +4,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <map>
 #include <set>
 
 namespace physisc {
@@ -32,7 +33,41 @@ const std::set<std::string> &LibraryTypeNames() {
 
 }  // namespace
 
+namespace {
+
+std::map<std::string, std::pair<int, std::string>> &UserGridTypes() {
+  static std::map<std::string, std::pair<int, std::string>> m;
+  return m;
+}
+
+std::map<std::string, std::vector<StructMember>> &Structs() {
+  static std::map<std::string, std::vector<StructMember>> m;
+  return m;
+}
+
+}  // namespace
+
+void RegisterUserGridType(const std::string &name, int dims,
+                          const std::string &elem) {
+  UserGridTypes()["PSGrid" + std::to_string(dims) + "D" + name] = {dims, elem};
+}
+
+bool IsUserGridType(const std::string &s) {
+  return UserGridTypes().count(s) > 0;
+}
+
+const std::vector<StructMember> *FindStruct(const std::string &type) {
+  auto it = Structs().find(type);
+  return it == Structs().end() ? nullptr : &it->second;
+}
+
 bool IsGridTypeName(const std::string &s, int *dims, std::string *elem) {
+  auto u = UserGridTypes().find(s);
+  if (u != UserGridTypes().end()) {
+    if (dims) *dims = u->second.first;
+    if (elem) *elem = u->second.second;
+    return true;
+  }
   // PSGrid{1,2,3}D{Float,Double,Int,Long}
   if (s.size() < 10 || s.compare(0, 6, "PSGrid") != 0) return false;
   char d = s[6];
@@ -116,13 +151,17 @@ std::string Parser::ParseBaseType() {
         ++pos_;
       }
       if (Peek().Is("{")) {  // inline body: keep it as text
-        int depth = 0;
-        do {
-          if (Peek().Is("{")) ++depth;
-          if (Peek().Is("}")) --depth;
-          ty += " " + Peek().text;
-          ++pos_;
-        } while (depth > 0 && Peek().kind != Tok::kEnd);
+        if (ty.compare(0, 6, "struct") == 0) {
+          ty += " " + ParseStructBody(ty);
+        } else {
+          int depth = 0;
+          do {
+            if (Peek().Is("{")) ++depth;
+            if (Peek().Is("}")) --depth;
+            ty += " " + Peek().text;
+            ++pos_;
+          } while (depth > 0 && Peek().kind != Tok::kEnd);
+        }
       }
       have_core = true;
       continue;
@@ -148,6 +187,64 @@ std::string Parser::ParseBaseType() {
   }
   if (ty.empty()) Fail("expected a type");
   return ty;
+}
+
+std::string Parser::ParseStructBody(const std::string &tag) {
+  // Members are declarations "type name[e1][e2], ...;" of scalar types with
+  // constant extents; anything else leaves the struct unregistered (it can
+  // still be used, but not as the point type of a grid).
+  std::string text;
+  size_t start = pos_;
+  Expect("{");
+  std::vector<StructMember> members;
+  bool plain = true;
+  while (!Peek().Is("}") && Peek().kind != Tok::kEnd) {
+    size_t p0 = pos_;
+    std::string base;
+    while (Peek().kind == Tok::kIdent && BuiltinTypeWords().count(Peek().text)) {
+      if (Peek().text != "const" && Peek().text != "volatile")
+        base += (base.empty() ? "" : " ") + Peek().text;
+      ++pos_;
+    }
+    if (base.empty() || base == "struct" || base == "union" || base == "enum") {
+      plain = false;
+      pos_ = p0;
+      int depth = 0;  // skip to the end of this member declaration
+      while (Peek().kind != Tok::kEnd && !(depth == 0 && Peek().Is(";"))) {
+        if (Peek().Is("{")) ++depth;
+        if (Peek().Is("}")) --depth;
+        ++pos_;
+      }
+      Accept(";");
+      continue;
+    }
+    do {
+      StructMember m;
+      m.type = base;
+      if (Peek().Is("*")) plain = false;
+      while (Peek().Is("*")) ++pos_;
+      if (Peek().kind != Tok::kIdent) Fail("expected a member name");
+      m.name = Peek().text;
+      ++pos_;
+      while (Accept("[")) {
+        const Token &n = Peek();
+        if (n.kind != Tok::kNumber) {
+          plain = false;
+          while (!Peek().Is("]") && Peek().kind != Tok::kEnd) ++pos_;
+        } else {
+          m.extents.push_back((int)std::strtol(n.text.c_str(), nullptr, 0));
+          ++pos_;
+        }
+        Expect("]");
+      }
+      members.push_back(m);
+    } while (Accept(","));
+    Expect(";");
+  }
+  Expect("}");
+  for (size_t i = start; i < pos_; ++i) text += (i > start ? " " : "") + t_[i].text;
+  if (plain && !members.empty() && tag != "struct") Structs()[tag] = members;
+  return text;
 }
 
 Declarator Parser::ParseDeclarator(bool allow_init) {
@@ -270,6 +367,9 @@ bool Parser::LooksLikeCast() const {
   if (a.kind != Tok::kIdent || !IsTypeName(a.text)) return false;
   // (T) or (T *...) followed by an operand
   size_t k = 2;
+  if ((a.text == "struct" || a.text == "union" || a.text == "enum") &&
+      Peek(2).kind == Tok::kIdent)
+    k = 3;  // the tag
   while (true) {
     const Token &b = Peek(k);
     if (b.Is(")")) return true;
@@ -570,6 +670,32 @@ Program Parser::ParseProgram() {
       ++pos_;
       continue;
     }
+    if (t.kind == Tok::kIdent && t.text.size() == 13 &&
+        t.text.compare(0, 11, "DeclareGrid") == 0 && t.text[12] == 'D' &&
+        t.text[11] >= '1' && t.text[11] <= '3' && Peek(1).Is("(")) {
+      // DeclareGrid<n>D(Name, type): declares the grid type PSGrid<n>DName
+      // of points of `type` (physis_user.h:27-44)
+      const int nd = t.text[11] - '0';
+      pos_ += 2;
+      if (Peek().kind != Tok::kIdent) Fail("expected a grid type name");
+      std::string name = Peek().text;
+      ++pos_;
+      Expect(",");
+      std::string elem;
+      while (!Peek().Is(")") && Peek().kind != Tok::kEnd) {
+        elem += (elem.empty() ? "" : " ") + Peek().text;
+        ++pos_;
+      }
+      Expect(")");
+      Accept(";");
+      RegisterUserGridType(name, nd, elem);
+      TopItem it;
+      it.kind = TopItem::kTokens;
+      it.tokens.assign(t_.begin() + start, t_.begin() + pos_);
+      it.user_grid = name;
+      prog.items.push_back(std::move(it));
+      continue;
+    }
     if (t.Is("typedef")) {
       // typedef ... name ;  (register the last identifier at depth 0)
       int depth = 0;
@@ -583,6 +709,25 @@ Program Parser::ParseProgram() {
       }
       ++pos_;
       typedefs_.insert(last);
+      // typedef struct [Tag] { ... } Name;  -> Name has Tag's members
+      if (Peek(0).kind != Tok::kEnd) {
+        size_t q = start + 1;
+        if (t_[q].Is("struct")) {
+          ++q;
+          std::string tag;
+          if (t_[q].kind == Tok::kIdent) tag = "struct " + t_[q++].text;
+          if (t_[q].Is("{")) {
+            size_t save = pos_;
+            pos_ = q;
+            ParseStructBody(tag.empty() ? "struct __anon" : tag);
+            pos_ = save;
+            if (const auto *m = FindStruct(tag.empty() ? "struct __anon" : tag))
+              Structs()[last] = *m;
+          } else if (!tag.empty() && FindStruct(tag)) {
+            Structs()[last] = *FindStruct(tag);
+          }
+        }
+      }
       TopItem it;
       it.kind = TopItem::kTokens;
       it.tokens.assign(t_.begin() + start, t_.begin() + pos_);

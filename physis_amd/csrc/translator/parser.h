@@ -11,8 +11,29 @@
 
 namespace physisc {
 
-// PSGrid{1,2,3}D{Float,Double,Int,Long}: dimensionality and C element type.
+// PSGrid{1,2,3}D{Float,Double,Int,Long} and the grid types a program
+// declares with DeclareGrid{1,2,3}D(Name, type): dimensionality and C element
+// type ("float", ..., or the user type's text, e.g. "struct Point").
 bool IsGridTypeName(const std::string &s, int *dims, std::string *elem);
+// A grid type declared by DeclareGrid<dims>D(name, elem) (PSGrid<dims>D<name>).
+void RegisterUserGridType(const std::string &name, int dims,
+                          const std::string &elem);
+bool IsUserGridType(const std::string &s);
+
+// Members of a struct defined in the program, in declaration order: scalar
+// base type, name and array extents (empty for a scalar member).
+struct StructMember {
+  std::string type;
+  std::string name;
+  std::vector<int> extents;
+  int Components() const {
+    int n = 1;
+    for (int e : extents) n *= e;
+    return n;
+  }
+};
+// "struct Tag" (or a typedef name) -> members; empty if unknown.
+const std::vector<StructMember> *FindStruct(const std::string &type);
 
 class Parser {
  public:
@@ -35,6 +56,9 @@ class Parser {
   static int BinaryPrec(const std::string &op);
 
   std::string ParseBaseType();
+  // struct body after '{' up to its '}' (registered under `tag`); returns
+  // the body's text
+  std::string ParseStructBody(const std::string &tag);
   std::string ParseTypeName();
   Declarator ParseDeclarator(bool allow_init);
   ExprP ParseInitList();

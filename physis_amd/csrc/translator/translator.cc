@@ -211,7 +211,20 @@ bool Translator::RegularIndex(const KernelInfo &k, const Expr &e, int axis,
 void Translator::AnalyzeGets(KernelInfo &k, const Expr &e) {
   if (const std::string *c = CalleeName(e)) {
     bool get = *c == "PSGridGet", getp = *c == "PSGridGetPeriodic";
-    if (get || getp || *c == "PSGridEmit" || *c == "PSGridDim") {
+    if (*c == "PSGridEmitUtype") {
+      // PSGridEmitUtype(g.member[i]..., v) (physis_user.h): the grid is the
+      // root of the member expression
+      if (e.kids.size() != 3) Fail(k.name, "PSGridEmitUtype takes two arguments");
+      const Expr *g = e.kids[1].get();
+      while (g->kind == EK::kIndex || g->kind == EK::kMember ||
+             g->kind == EK::kParen)
+        g = g->kids[0].get();
+      if (g->kind != EK::kIdent || k.GridIndex(g->text) < 0 ||
+          !k.grids[k.GridIndex(g->text)].user)
+        Fail(k.name, "PSGridEmitUtype on something that is not a member of a "
+                     "grid of user-defined points");
+      k.grids[k.GridIndex(g->text)].emitted = true;
+    } else if (get || getp || *c == "PSGridEmit" || *c == "PSGridDim") {
       const Expr *g = e.kids[1].get();
       if (g->kind != EK::kIdent || k.GridIndex(g->text) < 0)
         Fail(k.name, *c + " on something that is not a grid parameter");
@@ -284,6 +297,17 @@ void Translator::AnalyzeKernel(KernelInfo &k) {
       g.elem = elem;
       g.ps_type = PsTypeOf(elem);
       g.dev_view = "__PSGrid" + std::to_string(nd) + "D" + Cap(elem) + "_dev";
+      if (IsUserGridType(t)) {
+        g.user = true;
+        g.uname = t.substr(8);  // PSGrid<n>D<uname>
+        g.members = FindStruct(elem);
+        if (!g.members)
+          Fail(k.name, "grid " + g.name + ": the members of its point type " +
+                           elem + " are unknown (a struct of scalars and "
+                           "arrays of scalars is required)");
+        g.ps_type = "PS_USER";
+        g.dev_view = "__PSUdtView_" + g.uname;
+      }
       k.order.push_back({'g', (int)k.grids.size()});
       k.grids.push_back(g);
     } else {
@@ -549,6 +573,11 @@ void Translator::AnalyzeFusion(KernelInfo &k) {
 void Translator::Decide(KernelInfo &k) {
   k.fast = k.ndims == 3;
   if (!k.fast) k.slow_reason = "not a 3-D stencil";
+  for (auto &g : k.grids)
+    if (g.user && k.fast) {
+      k.fast = false;
+      k.slow_reason = "grid " + g.name + " holds user-defined points";
+    }
   if (k.used_rb) {
     k.fast = false;
     k.slow_reason = "red-black map";
@@ -615,11 +644,54 @@ void Translator::Decide(KernelInfo &k) {
 
 // ========================================================= device rewriting
 void Translator::RewriteKernelExpr(const KernelInfo &k, ExprP &e) {
+  if ((e->kind == EK::kIndex || e->kind == EK::kMember) &&
+      RewriteUserMemberGet(k, e))
+    return;
   for (auto &kid : e->kids)
     if (kid) RewriteKernelExpr(k, kid);
   const std::string *c = CalleeName(*e);
   if (!c) return;
   bool get = *c == "PSGridGet", getp = *c == "PSGridGetPeriodic";
+  if ((get || getp) && k.Grid(e->kids[1]->text)->user) {
+    // the whole point: every member loaded at the offset
+    const GridParam &g = *k.Grid(e->kids[1]->text);
+    e = Raw("__PSUdtLoad_" + g.uname + "(__ctx." + g.name + "_u, " +
+            UserOffset(k, g, *e) + ")");
+    return;
+  }
+  if (*c == "PSGridEmit" && k.Grid(e->kids[1]->text)->user) {
+    const GridParam &g = *k.Grid(e->kids[1]->text);
+    e = Raw("__PSUdtStore_" + g.uname + "(__ctx." + g.name + "_u, " +
+            UserPointOffset(k, g) + ", " + Print(*e->kids[2]) + ")");
+    return;
+  }
+  if (*c == "PSGridEmitUtype") {
+    // g.member[i]... = v at the current point
+    std::vector<ExprP *> idx;
+    Expr *x = e->kids[1].get();
+    while (x->kind == EK::kParen) x = x->kids[0].get();
+    while (x->kind == EK::kIndex) {
+      idx.insert(idx.begin(), &x->kids[1]);
+      x = x->kids[0].get();
+      while (x->kind == EK::kParen) x = x->kids[0].get();
+    }
+    if (x->kind != EK::kMember || x->op != ".")
+      Fail(k.name, "PSGridEmitUtype needs grid.member as its first argument");
+    const Expr *gx = x->kids[0].get();
+    while (gx->kind == EK::kParen) gx = gx->kids[0].get();
+    const GridParam &g = *k.Grid(gx->text);
+    const StructMember *m = nullptr;
+    for (auto &mm : *g.members)
+      if (mm.name == x->text) m = &mm;
+    if (!m) Fail(k.name, "point type of " + g.name + " has no member " + x->text);
+    std::string comp = UserComponent(k, g, *m, idx);
+    e = Raw("physis_hip::UdtStore<" + m->type + ">(__ctx." + g.name + "_u.m_" +
+            m->name + ", __ctx." + g.name + "_u.n, " +
+            std::to_string(m->Components()) + ", " + comp + ", " +
+            UserPointOffset(k, g) + ", (" + m->type + ")(" +
+            Print(*e->kids[2]) + "))");
+    return;
+  }
   if (get || getp) {
     const GridParam &g = *k.Grid(e->kids[1]->text);
     std::array<int, 3> o = {0, 0, 0};
@@ -654,9 +726,172 @@ void Translator::RewriteKernelExpr(const KernelInfo &k, ExprP &e) {
   }
   if (*c == "PSGridDim") {
     const std::string &n = e->kids[1]->text;
-    e = Raw("__ctx." + n + "_dim[" + Print(*e->kids[2]) + "]");
+    if (k.Grid(n) && k.Grid(n)->user)
+      e = Raw("__ctx." + n + "_u.dim[" + Print(*e->kids[2]) + "]");
+    else
+      e = Raw("__ctx." + n + "_dim[" + Print(*e->kids[2]) + "]");
     return;
   }
+}
+
+// Offset of PSGridGet(g, i...) / PSGridGetPeriodic on a user-point grid, its
+// index arguments already rewritten.
+std::string Translator::UserOffset(const KernelInfo &k, const GridParam &g,
+                                   const Expr &call) {
+  (void)k;
+  const bool per = *CalleeName(call) == "PSGridGetPeriodic";
+  if ((int)call.kids.size() - 2 != g.dims)
+    Fail(k.name, "get of " + g.name + " with the wrong index count");
+  std::string s = std::string("physis_hip::") +
+                  (per ? "OffsetPeriodic(" : "OffsetLinear(") + "__ctx." +
+                  g.name + "_u.dim, " + std::to_string(g.dims);
+  for (int a = 0; a < g.dims; ++a) s += ", " + Print(*call.kids[2 + a]);
+  return s + ")";
+}
+
+// Offset of the current point in a user-point grid (emits).
+std::string Translator::UserPointOffset(const KernelInfo &k,
+                                        const GridParam &g) const {
+  if (g.dims > k.ndims)
+    Fail(k.name, "emit into " + g.name + ", which has more dimensions than "
+                 "the stencil");
+  std::string s = "((long)" + k.index[0];
+  if (g.dims >= 2) s += " + (long)" + k.index[1] + " * __ctx." + g.name + "_u.dim[0]";
+  if (g.dims >= 3)
+    s += " + (long)" + k.index[2] + " * __ctx." + g.name + "_u.dim[0] * __ctx." +
+         g.name + "_u.dim[1]";
+  return s + ")";
+}
+
+// Row-major component number of member m indexed by idx (rewritten first).
+std::string Translator::UserComponent(const KernelInfo &k, const GridParam &g,
+                                      const StructMember &m,
+                                      std::vector<ExprP *> &idx) {
+  if (idx.size() != m.extents.size())
+    Fail(k.name, "member " + m.name + " of the points of " + g.name +
+                 " must be indexed down to a scalar");
+  std::string comp = "0";
+  for (size_t i = 0; i < idx.size(); ++i) {
+    RewriteKernelExpr(k, *idx[i]);
+    comp = i == 0 ? "(" + Print(**idx[i]) + ")"
+                  : "(" + comp + ") * " + std::to_string(m.extents[i]) + " + (" +
+                        Print(**idx[i]) + ")";
+  }
+  return comp;
+}
+
+// PSGridGet(g, i...).member[j]... (or PSGridGetPeriodic) on a grid of
+// user-defined points: a load of one component plane (the CUDA target's
+// member arrays, cuda_runtime_builder.cc:262-345).
+bool Translator::RewriteUserMemberGet(const KernelInfo &k, ExprP &e) {
+  std::vector<ExprP *> idx;
+  Expr *x = e.get();
+  while (x->kind == EK::kIndex) {
+    idx.insert(idx.begin(), &x->kids[1]);
+    x = x->kids[0].get();
+    while (x->kind == EK::kParen) x = x->kids[0].get();
+  }
+  if (x->kind != EK::kMember || x->op != ".") return false;
+  Expr *call = x->kids[0].get();
+  while (call->kind == EK::kParen) call = call->kids[0].get();
+  const std::string *c = CalleeName(*call);
+  if (!c || (*c != "PSGridGet" && *c != "PSGridGetPeriodic")) return false;
+  const GridParam *g = k.Grid(call->kids[1]->text);
+  if (!g || !g->user) return false;
+  const StructMember *m = nullptr;
+  for (auto &mm : *g->members)
+    if (mm.name == x->text) m = &mm;
+  if (!m) Fail(k.name, "point type of " + g->name + " has no member " + x->text);
+  // a partly indexed array member is an array value: leave it to the whole
+  // point load
+  if (idx.size() < m->extents.size()) return false;
+  for (size_t a = 2; a < call->kids.size(); ++a)
+    RewriteKernelExpr(k, call->kids[a]);
+  std::vector<ExprP *> own(idx.begin(), idx.begin() + m->extents.size());
+  std::string comp = UserComponent(k, *g, *m, own);
+  std::string off = UserOffset(k, *g, *call);
+  std::string r = "physis_hip::UdtLoad<" + m->type + ">(__ctx." + g->name +
+                  "_u.m_" + m->name + ", __ctx." + g->name + "_u.n, " +
+                  std::to_string(m->Components()) + ", " + comp + ", " + off +
+                  ")";
+  if (idx.size() > m->extents.size())
+    Fail(k.name, "member " + m->name + " indexed beyond its extents");
+  e = Raw(r);
+  return true;
+}
+
+std::string Translator::EmitUserType(const std::string &uname) {
+  int nd = 0;
+  std::string elem;
+  for (int d = 1; d <= 3 && elem.empty(); ++d)
+    if (IsGridTypeName("PSGrid" + std::to_string(d) + "D" + uname, &nd, &elem))
+      break;
+  const std::vector<StructMember> *ms = FindStruct(elem);
+  if (!ms) return "";  // never a stencil's point type: nothing to emit
+  std::ostringstream o;
+  const std::string V = "__PSUdtView_" + uname;
+  auto pstype = [](const std::string &t) {
+    if (t == "float") return "PS_FLOAT";
+    if (t == "double") return "PS_DOUBLE";
+    if (t == "int") return "PS_INT";
+    if (t == "long") return "PS_LONG";
+    return "PS_USER";
+  };
+  o << "// ---------------------------------------------- point type " << uname
+    << " (" << elem << ")\n"
+    << "// Type info of the reference's REF target (reference_runtime_builder.cc:\n"
+    << "// 959-1065): one entry per member, scalar size, array rank and extents.\n"
+    << "static __PSGridTypeMemberInfo __PSTypeMembers_" << uname << "[] = {\n";
+  for (auto &m : *ms) {
+    o << "  {" << pstype(m.type) << ", sizeof(" << m.type << "), "
+      << m.extents.size() << ", {";
+    for (size_t i = 0; i < m.extents.size(); ++i)
+      o << (i ? ", " : "") << m.extents[i];
+    if (m.extents.empty()) o << "0";
+    o << "}},\n";
+  }
+  o << "};\nstatic __PSGridTypeInfo __PSTypeInfo_" << uname << " = {PS_USER, "
+    << "sizeof(" << elem << "), " << ms->size() << ", __PSTypeMembers_" << uname
+    << "};\n\n";
+  o << "// Device view: the first component plane of each member (component c\n"
+    << "// of an array member, row-major, c * n elements further).\n"
+    << "struct " << V << " {\n";
+  for (auto &m : *ms) o << "  " << m.type << " *m_" << m.name << ";\n";
+  o << "  long n;\n  int dim[3];\n};\n\n";
+  o << "static inline " << V << " __PSUdtViewOf_" << uname << "(__PSGrid *g) {\n"
+    << "  " << V << " v;\n";
+  for (size_t i = 0; i < ms->size(); ++i)
+    o << "  v.m_" << (*ms)[i].name << " = (" << (*ms)[i].type
+      << " *)__PSGridUserMember(g, " << i << ");\n";
+  o << "  v.n = (long)g->num_elms;\n"
+    << "  for (int d = 0; d < 3; ++d) v.dim[d] = g->dim[d];\n  return v;\n}\n\n";
+  auto access = [](const StructMember &m, const std::string &var) {
+    std::string z;
+    for (size_t i = 0; i < m.extents.size(); ++i) z += "[0]";
+    return "(&" + var + "." + m.name + z + ")";
+  };
+  o << "__device__ __forceinline__ " << elem << " __PSUdtLoad_" << uname
+    << "(const " << V << " &v, long off) {\n  " << elem << " r{};\n"
+    << "  if (off < 0) return r;\n";
+  for (auto &m : *ms) {
+    if (m.extents.empty())
+      o << "  r." << m.name << " = v.m_" << m.name << "[off];\n";
+    else
+      o << "  for (int c = 0; c < " << m.Components() << "; ++c) "
+        << access(m, "r") << "[c] = v.m_" << m.name << "[c * v.n + off];\n";
+  }
+  o << "  return r;\n}\n\n";
+  o << "__device__ __forceinline__ void __PSUdtStore_" << uname << "(const " << V
+    << " &v, long off, const " << elem << " &r) {\n";
+  for (auto &m : *ms) {
+    if (m.extents.empty())
+      o << "  v.m_" << m.name << "[off] = r." << m.name << ";\n";
+    else
+      o << "  for (int c = 0; c < " << m.Components() << "; ++c) v.m_" << m.name
+        << "[c * v.n + off] = " << access(m, "r") << "[c];\n";
+  }
+  o << "}\n\n";
+  return o.str();
 }
 
 void Translator::RewriteKernelStmt(const KernelInfo &k, Stmt &s) {
@@ -767,6 +1002,10 @@ std::string Translator::EmitContext(const KernelInfo &k) {
       << (k.fusable ? std::string("yes") : "no (" + k.fuse_why + ")") << "\n";
   o << "struct __PSCtx_" << k.name << " {\n";
   for (auto &g : k.grids) {
+    if (g.user) {
+      o << "  " << g.dev_view << " " << g.name << "_u;\n";
+      continue;
+    }
     if (g.mode == GridParam::kStaged && k.fast)
       o << "  " << g.elem << " " << g.name << "[" << g.offsets.size() << "];\n";
     if (g.mode == GridParam::kDirect && k.fast)
@@ -910,6 +1149,9 @@ std::string Translator::EmitFastRunKernel(const KernelInfo &k) {
   o << "    __syncthreads();\n";
   o << "    __PSCtx_" << n << " ctx;\n";
   for (auto &g : k.grids)
+    if (g.user)
+      o << "    ctx." << g.name << "_u = " << g.name << ";\n";
+    else
     o << "    ctx." << g.name << "_p = " << g.name << ".p;\n"
       << "    for (int d = 0; d < 3; ++d) ctx." << g.name << "_dim[d] = "
       << g.name << ".dim[d];\n";
@@ -1041,6 +1283,9 @@ std::string Translator::EmitGenericRunKernel(const KernelInfo &k) {
   o << "  if (x < dom.local_max[0]" << (k.ndims >= 2 ? " && y < dom.local_max[1]" : "")
     << ") {\n    __PSCtx_" << n << " ctx;\n";
   for (auto &g : k.grids)
+    if (g.user)
+      o << "    ctx." << g.name << "_u = " << g.name << ";\n";
+    else
     o << "    ctx." << g.name << "_p = " << g.name << ".p;\n"
       << "    for (int d = 0; d < 3; ++d) ctx." << g.name << "_dim[d] = "
       << (g.dims == 1 ? "d < 1 ? " + g.name + ".dim[d] : 1"
@@ -1061,7 +1306,7 @@ std::string Translator::EmitGenericRunKernel(const KernelInfo &k) {
     o << "      if (__rb >= 0 && ((" << sum << " + __rb) & 1)) continue;\n";
   }
   for (auto &g : k.grids)
-    if (g.emitted) o << "      ctx." << g.name << "_e = false;\n";
+    if (g.emitted && !g.user) o << "      ctx." << g.name << "_e = false;\n";
   std::vector<std::string> args = {"x"};
   if (k.ndims >= 2) args.push_back("y");
   if (k.ndims >= 3) args.push_back("z");
@@ -1074,7 +1319,7 @@ std::string Translator::EmitGenericRunKernel(const KernelInfo &k) {
   else
     o << "      " << n << "(" << Join(args, ", ") << ");\n";
   for (auto &g : k.grids) {
-    if (!g.emitted) continue;
+    if (!g.emitted || g.user) continue;  // user points are stored in place
     std::string off = "x";
     if (g.dims >= 2) off += " + (long)y * " + g.name + ".dim[0]";
     if (g.dims >= 3)
@@ -1316,7 +1561,9 @@ std::string Translator::EmitLauncher(const KernelInfo &k) {
   const std::string RT = red ? k.ret_type : "float";
   std::vector<std::string> args = {"s.dom"};
   for (auto &g : k.grids)
-    args.push_back("physis_hip::DevView<" + g.dev_view + ">(s." + g.name + ")");
+    args.push_back(g.user ? "__PSUdtViewOf_" + g.uname + "(s." + g.name + ")"
+                          : "physis_hip::DevView<" + g.dev_view + ">(s." +
+                                g.name + ")");
   for (auto &s : k.scalars) args.push_back("s." + s.name);
   std::string A = Join(args, ", ");
   if (k.fast) {
@@ -1612,6 +1859,16 @@ void Translator::RewriteHostExpr(ExprP &e, const Function &f) {
   int nd;
   std::string elem;
   if (c.size() > 3 && c.compare(c.size() - 3, 3, "New") == 0 &&
+      IsUserGridType(c.substr(0, c.size() - 3))) {
+    IsGridTypeName(c.substr(0, c.size() - 3), &nd, &elem);
+    if (!FindStruct(elem))
+      Fail(f.name, c + ": the members of " + elem + " are unknown (a struct "
+                   "of scalars and arrays of scalars is required)");
+    std::string s = "physis_hip::NewUserGrid(&__PSTypeInfo_" + c.substr(8, c.size() - 11) +
+                    ", " + std::to_string(nd);
+    for (int a = 0; a < nd; ++a) s += ", " + Print(*e->kids[1 + a]);
+    e = Raw(s + ")");
+  } else if (c.size() > 3 && c.compare(c.size() - 3, 3, "New") == 0 &&
       IsGridTypeName(c.substr(0, c.size() - 3), &nd, &elem)) {
     std::string s = "physis_hip::NewGrid(" + PsTypeOf(elem) + ", sizeof(" +
                     elem + "), " + std::to_string(nd);
@@ -1689,12 +1946,20 @@ std::string Translator::Translate() {
     if (in_c) out << "}  // extern \"C\"\n\n";
     in_c = false;
   };
-  std::set<std::string> emitted_kernels;
+  std::set<std::string> emitted_kernels, user_types;
   for (auto &it : prog_.items) {
     if (it.kind == TopItem::kInclude) continue;
     if (it.kind == TopItem::kTokens) {
       open_c();
       out << PrintTokens(it.tokens) << "\n\n";
+      if (!it.user_grid.empty() && !user_types.count(it.user_grid)) {
+        user_types.insert(it.user_grid);
+        std::string u = EmitUserType(it.user_grid);
+        if (!u.empty()) {
+          close_c();
+          out << u;
+        }
+      }
       continue;
     }
     Function &f = *it.func;

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "ast.h"
+#include "parser.h"
 
 namespace physisc {
 
@@ -46,6 +47,12 @@ struct GridParam {
   bool periodic = false;    // staged tile wraps
   int hx = 0, hy = 0, zl = 0, zr = 0;
   enum Mode { kNone, kStaged, kDirect } mode = kNone;
+  // a grid of user-defined points (DeclareGrid<n>D(uname, elem)): stored as
+  // one plane per scalar member component, accessed through the view
+  // __PSUdtView_<uname>
+  bool user = false;
+  std::string uname;
+  const std::vector<StructMember> *members = nullptr;
 };
 
 struct ScalarParam {
@@ -125,6 +132,16 @@ class Translator {
   int RegEstimate(const KernelInfo &k, int pd) const;
   int PrefetchFor(const KernelInfo &k) const;
   std::string EmitRunFunction(const RunSite &r);
+  // type info, device view and load/store helpers of a user point type
+  std::string EmitUserType(const std::string &uname);
+  // PSGridGet(g, ...).member[i]... on a grid of user-defined points
+  bool RewriteUserMemberGet(const KernelInfo &k, ExprP &e);
+  std::string UserOffset(const KernelInfo &k, const GridParam &g,
+                         const Expr &call);
+  std::string UserPointOffset(const KernelInfo &k, const GridParam &g) const;
+  std::string UserComponent(const KernelInfo &k, const GridParam &g,
+                            const StructMember &m,
+                            std::vector<ExprP *> &idx);
 
   // host code
   void RewriteHostExpr(ExprP &e, const Function &f);

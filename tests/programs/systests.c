@@ -399,3 +399,139 @@ void systest_stencil_hole(float *out) {
   PSGridFree(g2);
   free(indata);
 }
+
+/* ---- user-defined point types */
+
+/* test_user-defined-type1.c:13-36: three scalar members, whole-point emit. */
+struct Vec3 {
+  float x;
+  float y;
+  float z;
+};
+DeclareGrid3D(Vec3, struct Vec3);
+
+static void kvec3(const int x, const int y, const int z, PSGrid3DVec3 g1,
+                  PSGrid3DVec3 g2) {
+  float p = (PSGridGet(g1, x, y, z).x +
+             PSGridGet(g1, x+1, y, z).x +
+             PSGridGet(g1, x-1, y, z).x) / 3.3f;
+  float q = (PSGridGet(g1, x, y, z).y +
+             PSGridGet(g1, x, y+1, z).y +
+             PSGridGet(g1, x, y-1, z).y +
+             PSGridGet(g1, x, y, z+1).y +
+             PSGridGet(g1, x, y, z-1).y) / 5.5f;
+  float r = (PSGridGet(g1, x, y, z).z +
+             PSGridGet(g1, x, y, z+1).z +
+             PSGridGet(g1, x, y, z-1).z) / 3.3f;
+  struct Vec3 v = {p, q, r};
+  PSGridEmit(g2, v);
+  return;
+}
+
+/* test_user-defined-type1.c main: points (i, i+1, i+2), 5 ping-pong pairs. */
+void systest_udt_vec3(struct Vec3 *out) {
+  size_t nelms = (size_t)N * N * N, i;
+  PSGrid3DVec3 g1 = PSGrid3DVec3New(N, N, N);
+  PSGrid3DVec3 g2 = PSGrid3DVec3New(N, N, N);
+  PSDomain3D d = PSDomain3DNew(1, N - 1, 1, N - 1, 1, N - 1);
+  struct Vec3 *indata = (struct Vec3 *)malloc(sizeof(struct Vec3) * nelms);
+  for (i = 0; i < nelms; i++) {
+    indata[i].x = i;
+    indata[i].y = i + 1;
+    indata[i].z = i + 2;
+  }
+  PSGridCopyin(g1, indata);
+  PSGridCopyin(g2, indata);
+  PSStencilRun(PSStencilMap(kvec3, d, g1, g2), PSStencilMap(kvec3, d, g2, g1),
+               5);
+  PSGridCopyout(g1, out);
+  PSGridFree(g1);
+  PSGridFree(g2);
+  free(indata);
+}
+
+/* test_user-defined-type-multi-dim-member.c:13-32: a 3x2 array member, one
+ * component emitted in place. */
+struct Tensor {
+  float p[3][2];
+};
+DeclareGrid3D(Tensor, struct Tensor);
+
+static void ktensor(const int x, const int y, const int z, PSGrid3DTensor g) {
+  float v1 = PSGridGet(g, x, y, z).p[0][0] +
+      PSGridGet(g, x-1, y, z).p[0][0] +
+      PSGridGet(g, x+1, y, z).p[0][0] +
+      PSGridGet(g, x, y-1, z).p[0][0] +
+      PSGridGet(g, x, y+1, z).p[0][0] +
+      PSGridGet(g, x, y, z-1).p[0][0] +
+      PSGridGet(g, x, y, z+1).p[0][0];
+  float v2 = PSGridGet(g, x, y, z).p[2][1] +
+      PSGridGet(g, x-1, y, z).p[2][1] +
+      PSGridGet(g, x+1, y, z).p[2][1] +
+      PSGridGet(g, x, y-1, z).p[2][1] +
+      PSGridGet(g, x, y+1, z).p[2][1] +
+      PSGridGet(g, x, y, z-1).p[2][1] +
+      PSGridGet(g, x, y, z+1).p[2][1];
+  PSGridEmitUtype(g.p[1][0], v1 + v2);
+  return;
+}
+
+/* test_user-defined-type-multi-dim-member.c main: p[j][k] = i + j + k. */
+void systest_udt_tensor(struct Tensor *out) {
+  int nelms = N * N * N, i, j, k;
+  PSGrid3DTensor g = PSGrid3DTensorNew(N, N, N);
+  PSDomain3D d = PSDomain3DNew(1, N - 1, 1, N - 1, 1, N - 1);
+  struct Tensor *indata = (struct Tensor *)malloc(sizeof(struct Tensor) * nelms);
+  for (i = 0; i < nelms; i++)
+    for (j = 0; j < 3; ++j)
+      for (k = 0; k < 2; ++k) indata[i].p[j][k] = i + j + k;
+  PSGridCopyin(g, indata);
+  PSStencilRun(PSStencilMap(ktensor, d, g));
+  PSGridCopyout(g, out);
+  PSGridFree(g);
+  free(indata);
+}
+
+/* test_user-defined-type-7-pt-periodic-complex.c:13-31: whole-point periodic
+ * gets. */
+struct Cplx {
+  float r;
+  float i;
+};
+DeclareGrid3D(Cplx, struct Cplx);
+
+static void kcplx(const int x, const int y, const int z, PSGrid3DCplx g1,
+                  PSGrid3DCplx g2) {
+  struct Cplx t = PSGridGetPeriodic(g1, x, y, z);
+  struct Cplx t1 = PSGridGetPeriodic(g1, x+1, y, z);
+  struct Cplx t2 = PSGridGetPeriodic(g1, x-1, y, z);
+  struct Cplx t3 = PSGridGetPeriodic(g1, x, y+1, z);
+  struct Cplx t4 = PSGridGetPeriodic(g1, x, y-1, z);
+  struct Cplx t5 = PSGridGetPeriodic(g1, x, y, z+1);
+  struct Cplx t6 = PSGridGetPeriodic(g1, x, y, z-1);
+  float r = t.r + t1.r + t2.r + t3.r + t4.r + t5.r + t6.r;
+  float i = t.i + t1.i + t2.i + t3.i + t4.i + t5.i + t6.i;
+  struct Cplx v = {r, i};
+  PSGridEmit(g2, v);
+  return;
+}
+
+/* test_user-defined-type-7-pt-periodic-complex.c main: (i, i+1), domain
+ * [0, N)^3. */
+void systest_udt_complex(struct Cplx *out) {
+  size_t nelms = (size_t)N * N * N, i;
+  PSGrid3DCplx g1 = PSGrid3DCplxNew(N, N, N);
+  PSGrid3DCplx g2 = PSGrid3DCplxNew(N, N, N);
+  PSDomain3D d = PSDomain3DNew(0, N, 0, N, 0, N);
+  struct Cplx *indata = (struct Cplx *)malloc(sizeof(struct Cplx) * nelms);
+  for (i = 0; i < nelms; i++) {
+    indata[i].r = i;
+    indata[i].i = i + 1;
+  }
+  PSGridCopyin(g1, indata);
+  PSStencilRun(PSStencilMap(kcplx, d, g1, g2));
+  PSGridCopyout(g2, out);
+  PSGridFree(g1);
+  PSGridFree(g2);
+  free(indata);
+}

@@ -57,9 +57,9 @@ CORPUS = [
 ]
 
 
-# Programs physisc-hip does not translate yet (user-defined point types): it
-# stops with an error naming the feature.
-UNSUPPORTED = {n for n in CORPUS if n.startswith("test_user-defined-type")}
+# Programs physisc-hip does not translate (none at present): it stops with an
+# error naming the feature.
+UNSUPPORTED = set()
 
 
 @pytest.mark.parametrize("name", CORPUS)

@@ -57,3 +57,37 @@ def test_reduction_3d_sum_matches_reference(prog):
     prog.systest_reduction_3d_sum(C.byref(v))
     assert sha256(("%f\n" % v.value).encode()) == \
         GOLD["test_reduction-3d-sum"]["sha256"]
+
+
+def _dump_rows(a):
+    """The reference dump of struct points: the members of each point as
+    "%f" separated by spaces, one point per line."""
+    return "".join(" ".join("%f" % v for v in row) + "\n"
+                   for row in a.tolist()).encode()
+
+
+def test_user_type_three_members(prog):
+    # test_user-defined-type1: dump "%f %f %f" of x, y, z
+    out = np.zeros((32 ** 3, 3), np.float32)
+    prog.systest_udt_vec3(out.ctypes.data_as(C.c_void_p))
+    assert sha256(_dump_rows(out)) == GOLD["test_user-defined-type1"]["sha256"]
+
+
+def test_user_type_array_member(prog):
+    # test_user-defined-type-multi-dim-member: dump of p[1][0]
+    out = np.zeros((32 ** 3, 3, 2), np.float32)
+    prog.systest_udt_tensor(out.ctypes.data_as(C.c_void_p))
+    assert sha256(dump_f(np.ascontiguousarray(out[:, 1, 0]))) == \
+        GOLD["test_user-defined-type-multi-dim-member"]["sha256"]
+    # the other components keep their inputs
+    i = np.arange(32 ** 3, dtype=np.float64)
+    for j, k in ((0, 0), (0, 1), (1, 1), (2, 0), (2, 1)):
+        assert np.array_equal(out[:, j, k], (i + j + k).astype(np.float32))
+
+
+def test_user_type_whole_point_periodic(prog):
+    # test_user-defined-type-7-pt-periodic-complex: dump "%f %f" of r, i
+    out = np.zeros((32 ** 3, 2), np.float32)
+    prog.systest_udt_complex(out.ctypes.data_as(C.c_void_p))
+    assert sha256(_dump_rows(out)) == \
+        GOLD["test_user-defined-type-7-pt-periodic-complex"]["sha256"]
