@@ -30,6 +30,10 @@
 namespace physis_hip {
 
 // ------------------------------------------------------ vector load / store
+// A compile-time flag passed to generic lambdas (the interior and general
+// forms of a wave step).
+template <bool B> struct Flag { static constexpr bool value = B; };
+
 template <typename T, int N> struct NativeVec { typedef T V; };
 template <> struct NativeVec<float, 2> {
   typedef float V __attribute__((ext_vector_type(2)));

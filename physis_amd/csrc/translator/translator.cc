@@ -110,6 +110,37 @@ bool IsMapName(const std::string &c, int *rb = nullptr) {
   return v != -2;
 }
 
+bool MentionsIdent(const Expr &e, const std::string &name) {
+  if (e.kind == EK::kIdent && e.text == name) return true;
+  for (const auto &k : e.kids)
+    if (k && MentionsIdent(*k, name)) return true;
+  return false;
+}
+
+// Whether a comparison (== != < > <= >=) in e mentions `name`.
+bool ComparesIdent(const Expr &e, const std::string &name) {
+  if (e.kind == EK::kBinary &&
+      (e.op == "==" || e.op == "!=" || e.op == "<" || e.op == ">" ||
+       e.op == "<=" || e.op == ">=") &&
+      MentionsIdent(e, name))
+    return true;
+  for (const auto &k : e.kids)
+    if (k && ComparesIdent(*k, name)) return true;
+  return false;
+}
+
+bool ComparesIdent(const Stmt &s, const std::string &name) {
+  if (s.expr && ComparesIdent(*s.expr, name)) return true;
+  if (s.cond && ComparesIdent(*s.cond, name)) return true;
+  if (s.step && ComparesIdent(*s.step, name)) return true;
+  if (s.init && ComparesIdent(*s.init, name)) return true;
+  for (const auto &d : s.decls)
+    if (d.init && ComparesIdent(*d.init, name)) return true;
+  for (const auto &k : s.kids)
+    if (k && ComparesIdent(*k, name)) return true;
+  return false;
+}
+
 }  // namespace
 
 Translator::Translator(
@@ -571,6 +602,8 @@ void Translator::AnalyzeFusion(KernelInfo &k) {
 }
 
 void Translator::Decide(KernelInfo &k) {
+  for (int a = 1; a < k.ndims; ++a)
+    k.compares_axis[a] = ComparesIdent(*k.fn->body, k.index[a]);
   k.fast = k.ndims == 3;
   if (!k.fast) k.slow_reason = "not a 3-D stencil";
   for (auto &g : k.grids)
@@ -1155,6 +1188,14 @@ std::string Translator::EmitFastRunKernel(const KernelInfo &k) {
     o << "    ctx." << g.name << "_p = " << g.name << ".p;\n"
       << "    for (int d = 0; d < 3; ++d) ctx." << g.name << "_dim[d] = "
       << g.name << ".dim[d];\n";
+  // VX > 1 runs only when every grid's x extent is a multiple of 4 and
+  // tiles start at multiples of TX: the compiler may use the low bits
+  o << "    if constexpr (VX > 1) {\n"
+       "      __builtin_assume((xs & (VX - 1)) == 0);\n";
+  for (auto &g : k.grids)
+    if (!g.user && (g.read || g.emitted))
+      o << "      __builtin_assume((ctx." << g.name << "_dim[0] & 3) == 0);\n";
+  o << "    }\n";
   o << "    for (int zc = zb; zc < ze; zc += PD) {\n#pragma unroll\n"
        "      for (int k = 0; k < PD; ++k) {\n"
        "        const int z = zc + k;\n"
@@ -1184,11 +1225,17 @@ std::string Translator::EmitFastRunKernel(const KernelInfo &k) {
     if (g.emitted)
       o << "          " << g.elem << " " << g.name << "_out[VX];\n"
         << "          unsigned " << g.name << "_mask = 0;\n";
-  o << "#pragma unroll\n          for (int i = 0; i < VX; ++i) {\n"
-       "            const int x = xs + i;\n";
-  for (auto &g : k.grids)
-    if (g.emitted) o << "            " << g.name << "_out[i] = 0;\n";
-  o << "            if (x >= dom.local_min[0] && x < dom.local_max[0]) {\n";
+  // Every point of the row is evaluated (staged and centre values are in
+  // registers, other gets are bounds-checked loads) and the domain test
+  // only masks the stores and the reduction, so the loop has no branches.
+  std::ostringstream b;
+  b << "#pragma unroll\n          for (int i = 0; i < VX; ++i) {\n"
+       "            const int x = xs + i;\n"
+       "            const bool __in = x >= dom.local_min[0] && x < dom.local_max[0];\n"
+       "            {\n";
+  std::ostringstream &ob = b;
+  {
+  std::ostringstream &o = ob;
   for (auto &g : k.grids) {
     if (g.mode == GridParam::kStaged) {
       auto rows = RowsOf(g);
@@ -1210,16 +1257,19 @@ std::string Translator::EmitFastRunKernel(const KernelInfo &k) {
   for (auto &s : k.scalars) args.push_back(s.name);
   if (red)
     o << "              const " << RT << " __r = " << n << "(" << Join(args, ", ")
-      << ");\n              if constexpr (RED >= 0) __racc = physis_hip::ReduceOp<"
-      << RT << ", RED < 0 ? 0 : RED>::apply(__racc, __r);\n";
+      << ");\n              if constexpr (RED >= 0) __racc = __in ? "
+         "physis_hip::ReduceOp<" << RT
+      << ", RED < 0 ? 0 : RED>::apply(__racc, __r) : __racc;\n";
   else
     o << "              " << n << "(" << Join(args, ", ") << ");\n";
   for (auto &g : k.grids)
     if (g.emitted)
-      o << "              if (ctx." << g.name << "_e) {\n                "
-        << g.name << "_out[i] = ctx." << g.name << "_v;\n                "
-        << g.name << "_mask |= 1u << i;\n              }\n";
+      o << "              " << g.name << "_out[i] = ctx." << g.name << "_v;\n"
+        << "              " << g.name << "_mask |= (__in && ctx." << g.name
+        << "_e) ? 1u << i : 0u;\n";
   o << "            }\n          }\n";
+  }
+  o << SplitInterior(k, "yrow", "z", b.str(), "          ");
   for (auto &g : k.grids)
     if (g.emitted)
       o << "          physis_hip::StoreMasked<" << g.elem << ", VX>(" << g.name
@@ -1347,6 +1397,59 @@ std::string Translator::EmitGenericRunKernel(const KernelInfo &k) {
 // (the halo columns x0-1 and x0+TX by the lanes of wave 0, which otherwise
 // only loads), and u''(s-2) from ring B on the tile's rows; one barrier per
 // step.
+std::string Translator::InteriorCond(const KernelInfo &k, const std::string &y,
+                                     const std::string &z) const {
+  std::vector<std::string> c;
+  const std::string *iv[3] = {nullptr, &y, &z};
+  for (int a = 1; a <= 2; ++a) {
+    if (!k.compares_axis[a]) continue;
+    for (auto &g : k.grids) {
+      if (g.user || g.dims != 3 || !(g.read || g.emitted)) continue;
+      c.push_back(*iv[a] + " >= 1 && " + *iv[a] + " <= ctx." + g.name + "_dim[" +
+                  std::to_string(a) + "] - 2");
+    }
+  }
+  return Join(c, " && ");
+}
+
+std::string Translator::InteriorAssume(const KernelInfo &k, const std::string &y,
+                                       const std::string &z) const {
+  std::string s;
+  const std::string *iv[3] = {nullptr, &y, &z};
+  for (int a = 1; a <= 2; ++a) {
+    if (!k.compares_axis[a]) continue;
+    for (auto &g : k.grids) {
+      if (g.user || g.dims != 3 || !(g.read || g.emitted)) continue;
+      const std::string d = "ctx." + g.name + "_dim[" + std::to_string(a) + "]";
+      s += "__builtin_assume(" + *iv[a] + " >= 1); __builtin_assume(" + *iv[a] +
+           " <= " + d + " - 2); __builtin_assume(" + *iv[a] +
+           " != 0); __builtin_assume(" + *iv[a] + " != " + d + " - 1); ";
+    }
+  }
+  return s;
+}
+
+// body: the element loop; it runs in two forms, the interior one under the
+// assumptions that y and z lie off the first and last planes of every grid
+// (so the compiler folds the body's boundary tests), chosen per wave step
+// by a wave-uniform branch.
+std::string Translator::SplitInterior(const KernelInfo &k, const std::string &y,
+                                      const std::string &z,
+                                      const std::string &body,
+                                      const std::string &ind) const {
+  const std::string cond = InteriorCond(k, y, z);
+  if (cond.empty()) return body;
+  std::ostringstream o;
+  o << ind << "auto __pts = [&](auto __interior) {\n"
+    << ind << "  if constexpr (decltype(__interior)::value) {\n"
+    << ind << "    " << InteriorAssume(k, y, z) << "\n" << ind << "  }\n"
+    << body << ind << "};\n"
+    << ind << "if (" << cond << ")\n"
+    << ind << "  __pts(physis_hip::Flag<true>{});\n"
+    << ind << "else\n" << ind << "  __pts(physis_hip::Flag<false>{});\n";
+  return o.str();
+}
+
 std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
   std::ostringstream o;
   const std::string &n = k.name;
@@ -1417,6 +1520,9 @@ std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
        "  const int xs = x0 + lane * VX;\n"
        "  const int nx = src.dim[0], nxy = src.dim[0] * src.dim[1];\n"
        "  const int n = nxy * src.dim[2];\n"
+       "  // tiles start at multiples of TX and two-sweep runs need nx % 4 == 0\n"
+       "  __builtin_assume((xs & (VX - 1)) == 0);\n"
+       "  __builtin_assume((nx & (VX - 1)) == 0);\n"
     << "  const physis_hip::BufWin<" << T << "> aw = physis_hip::BufWin<" << T
     << ">::Make(src.p, n, (long)(zb - 3) * nxy, (long)(ze + 3) * nxy);\n"
     << "  const physis_hip::BufWin<" << T << "> dw = physis_hip::BufWin<" << T
@@ -1441,12 +1547,16 @@ std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
        "      if (s > ze + 1) break;  // uniform over the block\n"
        "      if (mid_row && s <= ze) {\n";
   o << read_rows("a", "a_lds", "SA", "abase", "        ");
-  o << "        " << T << " uo[VX];\n"
-       "#pragma unroll\n        for (int i = 0; i < VX; ++i) {\n";
-  o << fill("a", "          ");
-  o << "          ctx." << O << "_e = false;\n"
-    << "          " << n << "(xs + i, yrow, s, " << fargs << ");\n"
-    << "          uo[i] = ctx." << O << "_v;\n        }\n";
+  o << "        " << T << " uo[VX];\n";
+  {
+    std::ostringstream b;
+    b << "#pragma unroll\n        for (int i = 0; i < VX; ++i) {\n";
+    b << fill("a", "          ");
+    b << "          ctx." << O << "_e = false;\n"
+      << "          " << n << "(xs + i, yrow, s, " << fargs << ");\n"
+      << "          uo[i] = ctx." << O << "_v;\n        }\n";
+    o << SplitInterior(k, "yrow", "s", b.str(), "        ");
+  }
   o << "        " << T << " *brow = b_lds + physis_hip::RingSlot<4>(bbase, 3) * "
        "SB::kPlane + wave * SB::kRow + SB::kPad;\n"
        "        *reinterpret_cast<physis_hip::NativeVec<" << T
@@ -1475,16 +1585,21 @@ std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
        "+ SB::kPad + px] = ctx." << O << "_v;\n      }\n";
   o << "      if (out_row && s - 2 >= zb) {\n";
   o << read_rows("b", "b_lds", "SB", "bbase", "        ");
-  o << "        " << T << " uu[VX];\n        unsigned m = 0;\n"
-       "#pragma unroll\n        for (int i = 0; i < VX; ++i) {\n"
-       "          uu[i] = 0;\n"
-       "          if (xs + i >= dom.local_min[0] && xs + i < dom.local_max[0]) {\n";
-  o << fill("b", "            ");
-  o << "            ctx." << O << "_e = false;\n"
-    << "            " << n << "(xs + i, yrow, s - 2, " << fargs << ");\n"
-    << "            uu[i] = ctx." << O << "_v;\n"
-    << "            m |= 1u << i;\n          }\n        }\n"
-    << "        physis_hip::StoreMasked<" << T
+  // every point of the row is computed (the values come from LDS); the
+  // store mask keeps the grid's columns
+  o << "        " << T << " uu[VX];\n        unsigned m = 0;\n";
+  {
+    std::ostringstream b;
+    b << "#pragma unroll\n        for (int i = 0; i < VX; ++i) {\n";
+    b << fill("b", "          ");
+    b << "          ctx." << O << "_e = false;\n"
+      << "          " << n << "(xs + i, yrow, s - 2, " << fargs << ");\n"
+      << "          uu[i] = ctx." << O << "_v;\n"
+      << "          m |= (xs + i >= dom.local_min[0] && xs + i < dom.local_max[0]) "
+         "? 1u << i : 0u;\n        }\n";
+    o << SplitInterior(k, "yrow", "s - 2", b.str(), "        ");
+  }
+  o << "        physis_hip::StoreMasked<" << T
     << ", VX>(dw, xs + yrow * nx + (s - 2) * nxy, uu, m);\n      }\n"
     << "      st[k].Store(a_lds + physis_hip::RingSlot<4>(abase, 3) * SA::kPlane, "
        "wave, lane);\n"

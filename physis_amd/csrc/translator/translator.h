@@ -81,6 +81,10 @@ struct KernelInfo {
   bool fusable = false;
   int fuse_in = -1, fuse_out = -1;  // grid indices of the read and emitted grids
   std::string fuse_why;             // why not, for the analysis comment
+  // the body compares index parameter y / z (axis 1 / 2) with something:
+  // run kernels then split each wave step into an interior form (y, z
+  // known to lie off the grid's first and last planes) and a general one
+  bool compares_axis[3] = {false, false, false};
   int GridIndex(const std::string &n) const {
     for (size_t i = 0; i < grids.size(); ++i)
       if (grids[i].name == n) return (int)i;
@@ -127,6 +131,17 @@ class Translator {
   std::string EmitGenericRunKernel(const KernelInfo &k);
   std::string EmitLauncher(const KernelInfo &k);
   std::string EmitFusedRunKernel(const KernelInfo &k);
+  // Interior split of a wave step at rows y, plane z: the wave-uniform
+  // condition and the facts assumed under it ("" when the body compares
+  // neither y nor z).
+  std::string InteriorCond(const KernelInfo &k, const std::string &y,
+                           const std::string &z) const;
+  std::string InteriorAssume(const KernelInfo &k, const std::string &y,
+                             const std::string &z) const;
+  // the lambda-wrapped element loop `body` called in its two forms
+  std::string SplitInterior(const KernelInfo &k, const std::string &y,
+                            const std::string &z, const std::string &body,
+                            const std::string &ind) const;
   // VGPRs per lane of the run kernel at VX = 4 with pd staged planes in
   // flight (a rough estimate), and the prefetch depth used for k.
   int RegEstimate(const KernelInfo &k, int pd) const;
