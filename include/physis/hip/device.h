@@ -124,6 +124,15 @@ __device__ __forceinline__ int WrapIndex(int i, int n) {
   return r < 0 ? r + n : r;
 }
 
+// WrapIndex for indices near the grid ([-n, 2n) without a division; others
+// fall back to it): plane and row indices of periodic stencils.
+__device__ __forceinline__ int WrapNear(int i, int n) {
+  if (i >= 0 && i < n) return i;
+  if (i < 0 && i >= -n) return i + n;
+  if (i >= n && i - n < n) return i - n;
+  return WrapIndex(i, n);
+}
+
 }  // namespace physis_hip
 
 #endif  // PHYSIS_HIP_DEVICE_H_

@@ -274,29 +274,57 @@ struct Stage {
     return x + y * dim[0] + z * dim[0] * dim[1];
   }
 
+  // The part of a row's addresses that does not change from plane to
+  // plane: the x offsets of the lane's VX points and of its halo point
+  // (lanes 0 .. 2*HX-1 carry the halo, the others repeat lane 0's) and the
+  // row's y offset, wrapped when periodic.  Computed once per row, so that
+  // a plane step only adds the plane offset.
+  struct Addr {
+    int xo, ho, yo;
+  };
+  __device__ __forceinline__ static Addr Prepare(const int *dim, int x0, int y,
+                                                 int lane) {
+    const int hl = lane < 2 * HX ? lane : 0;
+    const int hx = hl < HX ? x0 - HX + hl : x0 + kTX + (hl - HX);
+    Addr a;
+    if (PERIODIC) {
+      a.xo = WrapIndex(x0 + lane * VX, dim[0]);
+      a.ho = WrapIndex(hx, dim[0]);
+      a.yo = WrapIndex(y, dim[1]) * dim[0];
+    } else {
+      a.xo = x0 + lane * VX;
+      a.ho = hx;
+      a.yo = y * dim[0];
+    }
+    return a;
+  }
+
   // Issues the two loads of plane z for the wave's row y = y0 - HY + wave.
   // Non-periodic reads go through the block's window w (the REF linear rule
   // may spill into the neighbouring plane); periodic reads go through a
   // window of exactly the wrapped plane, which may lie anywhere in the grid.
   __device__ __forceinline__ void Load(const BufWin<M> &w, const M *p,
-                                       const int *dim, int x0, int y, int z,
-                                       int lane) {
-    const int hl = lane < 2 * HX ? lane : 0;
-    const int hx = hl < HX ? x0 - HX + hl : x0 + kTX + (hl - HX);
+                                       const int *dim, const Addr &a, int z) {
     M v[1];
     if (PERIODIC) {
       const int nxy = dim[0] * dim[1];
-      const int zw = WrapIndex(z, dim[2]);
+      const int zw = WrapNear(z, dim[2]);
       const BufWin<M> pw =
           BufWin<M>::Make(p, nxy * dim[2], (long)zw * nxy, (long)(zw + 1) * nxy);
-      const int yw = WrapIndex(y, dim[1]) * dim[0] + zw * nxy;
-      pw.template Load<VX>(WrapIndex(x0 + lane * VX, dim[0]) + yw, own);
-      pw.template Load<1>(WrapIndex(hx, dim[0]) + yw, v);
+      const int zo = zw * nxy;
+      pw.template Load<VX>(a.xo + a.yo + zo, own);
+      pw.template Load<1>(a.ho + a.yo + zo, v);
     } else {
-      w.template Load<VX>(Offset(dim, x0 + lane * VX, y, z), own);
-      w.template Load<1>(Offset(dim, hx, y, z), v);
+      const int zo = z * dim[0] * dim[1];
+      w.template Load<VX>(a.xo + a.yo + zo, own);
+      w.template Load<1>(a.ho + a.yo + zo, v);
     }
     halo = v[0];
+  }
+  __device__ __forceinline__ void Load(const BufWin<M> &w, const M *p,
+                                       const int *dim, int x0, int y, int z,
+                                       int lane) {
+    Load(w, p, dim, Prepare(dim, x0, y, lane), z);
   }
 
   // Writes the wave's row of the plane into one LDS slot.

@@ -1150,14 +1150,16 @@ std::string Translator::EmitFastRunKernel(const KernelInfo &k) {
         << G << "_nxy * " << G << ".dim[2], (long)(zb + " << g.zl - 1 << ") * "
         << G << "_nxy, (long)(ze + " << g.zr + 1 << ") * " << G << "_nxy);\n";
       o << "    S_" << G << " " << G << "_st[PD];\n";
+      o << "    const typename S_" << G << "::Addr " << G << "_ra = S_" << G
+        << "::Prepare(" << G << ".dim, x0, yrow, lane);\n";
       o << "#pragma unroll\n    for (int k = 0; k < " << nslot - 1 << "; ++k) {\n"
         << "      " << G << "_st[0].Load(" << G << "_w, " << G << ".p, " << G
-        << ".dim, x0, yrow, zb + " << g.zl << " + k, lane);\n"
+        << ".dim, " << G << "_ra, zb + " << g.zl << " + k);\n"
         << "      " << G << "_st[0].Store(" << G << "_lds + k * S_" << G
         << "::kPlane, wave, lane);\n    }\n";
       o << "#pragma unroll\n    for (int k = 0; k < PD; ++k)\n      " << G
-        << "_st[k].Load(" << G << "_w, " << G << ".p, " << G
-        << ".dim, x0, yrow, zb + " << g.zr + 1 << " + k, lane);\n";
+        << "_st[k].Load(" << G << "_w, " << G << ".p, " << G << ".dim, " << G
+        << "_ra, zb + " << g.zr + 1 << " + k);\n";
       o << "    int " << G << "_base = 0;\n";
     } else if (g.mode == GridParam::kDirect) {
       o << "    const int " << G << "_nx = " << G << ".dim[0], " << G
@@ -1288,8 +1290,8 @@ std::string Translator::EmitFastRunKernel(const KernelInfo &k) {
     if (g.mode != GridParam::kStaged) continue;
     int nslot = g.zr - g.zl + 2;
     o << "        " << g.name << "_st[k].Load(" << g.name << "_w, " << g.name
-      << ".p, " << g.name << ".dim, x0, yrow, z + " << g.zr + 1
-      << " + PD, lane);\n"
+      << ".p, " << g.name << ".dim, " << g.name << "_ra, z + " << g.zr + 1
+      << " + PD);\n"
       << "        " << g.name << "_base = physis_hip::RingSlot<" << nslot
       << ">(" << g.name << "_base, 1);\n";
   }
@@ -1530,11 +1532,12 @@ std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
     << "  const physis_hip::BufWin<" << T << "> mw = physis_hip::BufWin<" << T
     << ">::Make(mid ? mid : dst.p, n, (long)zb * nxy, (long)ze * nxy);\n"
     << "  SA st[PD];\n"
+       "  const typename SA::Addr ra = SA::Prepare(src.dim, x0, yrow, lane);\n"
        "#pragma unroll\n  for (int k = 0; k < 3; ++k) {\n"
-       "    st[0].Load(aw, src.p, src.dim, x0, yrow, zb - 2 + k, lane);\n"
+       "    st[0].Load(aw, src.p, src.dim, ra, zb - 2 + k);\n"
        "    st[0].Store(a_lds + k * SA::kPlane, wave, lane);\n  }\n"
        "#pragma unroll\n  for (int k = 0; k < PD; ++k)\n"
-       "    st[k].Load(aw, src.p, src.dim, x0, yrow, zb + 1 + k, lane);\n"
+       "    st[k].Load(aw, src.p, src.dim, ra, zb + 1 + k);\n"
        "  int abase = 0, bbase = 0;\n"
        "  __syncthreads();\n"
     << "  __PSCtx_" << n << " ctx;\n"
@@ -1604,7 +1607,7 @@ std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
     << "      st[k].Store(a_lds + physis_hip::RingSlot<4>(abase, 3) * SA::kPlane, "
        "wave, lane);\n"
        "      __syncthreads();\n"
-       "      st[k].Load(aw, src.p, src.dim, x0, yrow, s + 2 + PD, lane);\n"
+       "      st[k].Load(aw, src.p, src.dim, ra, s + 2 + PD);\n"
        "      abase = physis_hip::RingSlot<4>(abase, 1);\n"
        "      bbase = physis_hip::RingSlot<4>(bbase, 1);\n"
        "    }\n  }\n}\n\n";
