@@ -1198,11 +1198,29 @@ std::string Translator::EmitFastRunKernel(const KernelInfo &k) {
     if (!g.user && (g.read || g.emitted))
       o << "      __builtin_assume((ctx." << g.name << "_dim[0] & 3) == 0);\n";
   o << "    }\n";
-  o << "    for (int zc = zb; zc < ze; zc += PD) {\n#pragma unroll\n"
-       "      for (int k = 0; k < PD; ++k) {\n"
+  // The z loop is unrolled over the prefetch depth.  When that is a
+  // multiple of every ring's slot count, ring slots are compile-time
+  // indices (each ring's oldest plane is in slot k % NSLOT at step k of the
+  // body); unrolling further to get there costs more registers than it
+  // saves (measured on the 19-point kernel), so other rings keep a running
+  // base.
+  const int U = PrefetchFor(k);
+  bool const_slots = true;
+  for (auto &g : k.grids)
+    if (g.mode == GridParam::kStaged && U % (g.zr - g.zl + 2)) const_slots = false;
+  o << "    constexpr int U = " << U << ";\n";
+  for (auto &g : k.grids)
+    if (g.mode == GridParam::kStaged && const_slots)
+      o << "    (void)" << g.name << "_base;\n";
+  o << "    for (int zc = zb; zc < ze; zc += U) {\n#pragma unroll\n"
+       "      for (int k = 0; k < U; ++k) {\n"
        "        const int z = zc + k;\n"
        "        const int cur = k & 1, nxt = cur ^ 1;\n"
        "        (void)cur; (void)nxt;\n";
+  for (auto &g : k.grids)
+    if (g.mode == GridParam::kStaged && const_slots)
+      o << "        const int " << g.name << "_base = k % " << g.zr - g.zl + 2
+        << ";\n";
   for (auto &g : k.grids)
     if (g.mode == GridParam::kDirect)
       o << "        " << g.name << "_w.template Load<VX>(xs + drow * " << g.name
@@ -1281,7 +1299,7 @@ std::string Translator::EmitFastRunKernel(const KernelInfo &k) {
   for (auto &g : k.grids) {
     if (g.mode != GridParam::kStaged) continue;
     int nslot = g.zr - g.zl + 2;
-    o << "        " << g.name << "_st[k].Store(" << g.name
+    o << "        " << g.name << "_st[k % PD].Store(" << g.name
       << "_lds + physis_hip::RingSlot<" << nslot << ">(" << g.name << "_base, "
       << nslot - 1 << ") * S_" << g.name << "::kPlane, wave, lane);\n";
   }
@@ -1289,11 +1307,12 @@ std::string Translator::EmitFastRunKernel(const KernelInfo &k) {
   for (auto &g : k.grids) {
     if (g.mode != GridParam::kStaged) continue;
     int nslot = g.zr - g.zl + 2;
-    o << "        " << g.name << "_st[k].Load(" << g.name << "_w, " << g.name
+    o << "        " << g.name << "_st[k % PD].Load(" << g.name << "_w, " << g.name
       << ".p, " << g.name << ".dim, " << g.name << "_ra, z + " << g.zr + 1
-      << " + PD);\n"
-      << "        " << g.name << "_base = physis_hip::RingSlot<" << nslot
-      << ">(" << g.name << "_base, 1);\n";
+      << " + PD);\n";
+    if (!const_slots)
+      o << "        " << g.name << "_base = physis_hip::RingSlot<" << nslot
+        << ">(" << g.name << "_base, 1);\n";
   }
   o << "      }\n    }\n  }\n";
   if (red)
@@ -1538,15 +1557,18 @@ std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
        "    st[0].Store(a_lds + k * SA::kPlane, wave, lane);\n  }\n"
        "#pragma unroll\n  for (int k = 0; k < PD; ++k)\n"
        "    st[k].Load(aw, src.p, src.dim, ra, zb + 1 + k);\n"
-       "  int abase = 0, bbase = 0;\n"
+
        "  __syncthreads();\n"
     << "  __PSCtx_" << n << " ctx;\n"
     << "  ctx." << I << "_p = src.p;\n  ctx." << O << "_p = dst.p;\n"
     << "  for (int d = 0; d < 3; ++d) {\n    ctx." << I << "_dim[d] = src.dim[d];\n"
     << "    ctx." << O << "_dim[d] = dst.dim[d];\n  }\n"
-    << "  for (int sc = zb - 1; sc <= ze + 1; sc += PD) {\n#pragma unroll\n"
-       "    for (int k = 0; k < PD; ++k) {\n"
+    // four steps per iteration: both rings' oldest planes are in slot k
+    // at step k, so slots are compile-time indices
+    << "  for (int sc = zb - 1; sc <= ze + 1; sc += 4) {\n#pragma unroll\n"
+       "    for (int k = 0; k < 4; ++k) {\n"
        "      const int s = sc + k;\n"
+       "      const int abase = k, bbase = k;\n"
        "      if (s > ze + 1) break;  // uniform over the block\n"
        "      if (mid_row && s <= ze) {\n";
   o << read_rows("a", "a_lds", "SA", "abase", "        ");
@@ -1604,12 +1626,10 @@ std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
   }
   o << "        physis_hip::StoreMasked<" << T
     << ", VX>(dw, xs + yrow * nx + (s - 2) * nxy, uu, m);\n      }\n"
-    << "      st[k].Store(a_lds + physis_hip::RingSlot<4>(abase, 3) * SA::kPlane, "
+    << "      st[k % PD].Store(a_lds + physis_hip::RingSlot<4>(abase, 3) * SA::kPlane, "
        "wave, lane);\n"
        "      __syncthreads();\n"
-       "      st[k].Load(aw, src.p, src.dim, ra, s + 2 + PD);\n"
-       "      abase = physis_hip::RingSlot<4>(abase, 1);\n"
-       "      bbase = physis_hip::RingSlot<4>(bbase, 1);\n"
+       "      st[k % PD].Load(aw, src.p, src.dim, ra, s + 2 + PD);\n"
        "    }\n  }\n}\n\n";
   // launcher
   std::vector<std::string> largs = {"s.dom", "src", "dst", "mid"};
