@@ -148,9 +148,13 @@ inline void CheckEmitDomain(const __PSDomain &dom, const __PSGrid *g) {
   }
 }
 
-// Brackets a run function: times it with HIP events when tracing is on and
-// always waits for the stream at the end, as the reference run function
-// does after its launch loop (cuda_runtime_builder.cc:1490-1507).
+// Brackets a run function.  Launches stay asynchronous, as in the
+// reference's run functions (cuda_runtime_builder.cc:1465-1507: they wait
+// for the device only under CUDA_KERNEL_ERROR_CHECK, which the translator's
+// kernel_error_check option mirrors with __PSCheckHipError); launch errors
+// are checked here, faults inside a kernel surface at the next synchronising
+// call (copy-out, reduction, free).  With tracing on (--physis-trace) the
+// run is timed with HIP events, which waits for it.
 class RunTimer {
  public:
   explicit RunTimer(hipStream_t stream) : stream_(stream) {
@@ -162,11 +166,11 @@ class RunTimer {
   }
   float Stop() {
     float ms = 0.0f;
+    hipError_t e = hipGetLastError();
     if (__ps_trace) {
       (void)hipEventRecord(stop_, stream_);
+      if (e == hipSuccess) e = hipStreamSynchronize(stream_);
     }
-    hipError_t e = hipStreamSynchronize(stream_);
-    if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) {
       std::fprintf(stderr, "[physis-hip] stencil run failed: %s\n",
                    hipGetErrorString(e));

@@ -139,7 +139,11 @@ def ktrace_md(kt_dir: str, out_md: str, command: str) -> None:
              "| kernel | calls | avg us | min us | max us | % time |",
              "|---|---|---|---|---|---|"]
     for n, c, a, lo, hi, pct in sorted(rows, key=lambda r: -r[5]):
-        name = n.split("(")[0].replace("|", "/")
+        # drop the parameter list (the last top-level parenthesis group);
+        # names may contain "(anonymous namespace)"
+        name = n.replace("(anonymous namespace)", "{anon}")
+        name = name.split("(")[0].replace("{anon}", "(anonymous namespace)")
+        name = name.replace("|", "/")
         lines.append(f"| `{name}` | {c} | {a/1e3:.1f} | {lo/1e3:.1f} | "
                      f"{hi/1e3:.1f} | {pct:.2f} |")
     lines.append("")
