@@ -124,3 +124,71 @@ def test_reference_corpus_translates():
             assert p.returncode != 0, name
         else:
             assert p.returncode == 0, (name, p.stderr)
+
+
+UDT_PROG = """#include "physis/physis.h"
+struct Cell {
+  float u;
+  float m[2][3];
+  double w;
+};
+DeclareGrid3D(Cell, struct Cell);
+static void k(const int x, const int y, const int z, PSGrid3DCell g,
+              PSGrid3DCell h) {
+  int i;
+  float s = PSGridGet(g, x + 1, y, z).u + PSGridGet(g, x, y, z).m[1][2];
+  for (i = 0; i < 2; ++i) s += PSGridGetPeriodic(g, x, y - 1, z).m[i][0];
+  struct Cell c = PSGridGet(g, x, y, z);
+  c.u = s;
+  PSGridEmit(h, c);
+  PSGridEmitUtype(g.w, (double)s);
+}
+void run(PSGrid3DCell g, PSGrid3DCell h) {
+  PSDomain3D d = PSDomain3DNew(1, 7, 1, 7, 1, 7);
+  PSStencilRun(PSStencilMap(k, d, g, h));
+}
+void make(void) {
+  PSGrid3DCell g = PSGrid3DCellNew(8, 8, 8);
+  PSGridFree(g);
+}
+"""
+
+
+def test_user_point_type_lowering():
+    rc, code, err = translate(None, UDT_PROG)
+    assert rc == 0, err
+    # type info as the REF target builds it: scalar size, rank, extents
+    assert "{PS_FLOAT, sizeof(float), 2, {2, 3}}" in code
+    assert "{PS_DOUBLE, sizeof(double), 0, {0}}" in code
+    assert "__PSTypeInfo_Cell = {PS_USER, sizeof(struct Cell), 3" in code
+    # one plane per scalar component: member gets index component planes
+    assert "physis_hip::UdtLoad<float>(__ctx.g_u.m_m, __ctx.g_u.n, 6, " \
+        "((1)) * 3 + (2)" in code
+    assert "((i)) * 3 + (0)" in code and "OffsetPeriodic" in code
+    # whole-point load / store and a member emit at the current point
+    assert "__PSUdtLoad_Cell(__ctx.g_u, physis_hip::OffsetLinear(" in code
+    assert "__PSUdtStore_Cell(__ctx.h_u, " in code
+    assert "physis_hip::UdtStore<double>(__ctx.g_u.m_w, __ctx.g_u.n, 1, 0, " in code
+    # grids of the type are created from the type info
+    assert "physis_hip::NewUserGrid(&__PSTypeInfo_Cell, 3, 8, 8, 8)" in code
+    assert "generic run kernel: grid g holds user-defined points" in code
+
+
+def test_user_point_member_must_be_indexed_to_a_scalar():
+    bad = UDT_PROG.replace("PSGridEmitUtype(g.w, (double)s);",
+                           "PSGridEmitUtype(g.m[0], s);")
+    rc, _, err = translate(None, bad)
+    assert rc != 0 and "indexed down to a scalar" in err
+
+
+def test_interior_split_only_for_kernels_comparing_y_or_z():
+    d7 = open(SOURCES[0][:-2] + ".hip.cpp").read()
+    him = open(SOURCES[1][:-2] + ".hip.cpp").read()
+    # the clamped diffusion compares y and z with the grid's first and last
+    # planes: its wave steps have an interior form
+    assert "__builtin_assume(yrow != ctx.g1_dim[1] - 1)" in d7
+    assert "__pts(physis_hip::Flag<true>{})" in d7
+    # Himeno never compares an index
+    assert "Flag<true>" not in him
+    # x-alignment facts of the VX = 4 form
+    assert "__builtin_assume((xs & (VX - 1)) == 0);" in d7
