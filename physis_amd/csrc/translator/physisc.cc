@@ -8,9 +8,20 @@
 // (the Physis header is replaced by the HIP target's); user macros are
 // expanded by the C preprocessor.
 //
-// Configuration file (KEY = value per line; cf. translator/configuration.h:
-// 16-36): HIP_BLOCK_WAVES, HIP_PREFETCH, HIP_KERNEL_ERROR_CHECK,
-// TRACE_KERNEL.
+// Configuration file: the reference's format (KEY = value per line, `--`
+// comments, Lua tables whose first entry is used, as the reference does when
+// not auto-tuning: translator/configuration.cc:27-60).  Keys:
+//   HIP_BLOCK_WAVES, HIP_PREFETCH     launch shape of the run kernels
+//   HIP_KERNEL_ERROR_CHECK, CUDA_KERNEL_ERROR_CHECK
+//                                     wait and check after each run
+//   TRACE_KERNEL                      trace output of run functions
+//   HIP_TWO_SWEEP                     two-sweep fusion of map pairs (true)
+//   OPT_LOOP_PEELING                  interior form of each wave step (true)
+// and, accepted with no effect because the HIP run kernel always does what
+// they ask (translator/optimizer/*.cc): OPT_KERNEL_INLINING,
+// OPT_REGISTER_BLOCKING, OPT_UNCONDITIONAL_GET, OPT_OFFSET_CSE,
+// OPT_OFFSET_SPATIAL_CSE, OPT_OFFSET_COMP, OPT_LOOP_OPT; CUDA_BLOCK_SIZE is
+// a CUDA launch shape and is ignored.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -62,11 +73,26 @@ void LoadConfig(const std::string &path, Config *cfg) {
     size_t eq = line.find('=');
     if (eq == std::string::npos) continue;
     std::string k = Trim(line.substr(0, eq)), v = Trim(line.substr(eq + 1));
+    // a Lua table of alternatives: its first entry
+    while (!v.empty() && v[0] == '{') {
+      v = Trim(v.substr(1));
+      size_t e = v.find_first_of(",}");
+      if (v.empty() || v[0] == '{') continue;
+      v = Trim(v.substr(0, e));
+    }
     bool on = v == "true" || v == "1";
     if (k == "HIP_BLOCK_WAVES") cfg->block_waves = std::atoi(v.c_str());
     else if (k == "HIP_PREFETCH") cfg->prefetch = std::atoi(v.c_str());
-    else if (k == "HIP_KERNEL_ERROR_CHECK") cfg->kernel_error_check = on;
+    else if (k == "HIP_KERNEL_ERROR_CHECK" || k == "CUDA_KERNEL_ERROR_CHECK")
+      cfg->kernel_error_check = on;
     else if (k == "TRACE_KERNEL") cfg->trace = on;
+    else if (k == "HIP_TWO_SWEEP") cfg->two_sweep = on;
+    else if (k == "OPT_LOOP_PEELING") cfg->interior_split = on;
+    else if (k.compare(0, 4, "OPT_") == 0 || k == "CUDA_BLOCK_SIZE")
+      continue;  // see the header comment
+    else
+      std::fprintf(stderr, "physisc-hip: %s: unknown configuration key %s\n",
+                   path.c_str(), k.c_str());
   }
   if (cfg->prefetch != 0 && (cfg->prefetch < 2 || cfg->prefetch % 2))
     cfg->prefetch = 2;

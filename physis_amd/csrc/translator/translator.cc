@@ -561,6 +561,7 @@ struct FuseScan {
 void Translator::AnalyzeFusion(KernelInfo &k) {
   k.fusable = false;
   auto no = [&](const std::string &w) { k.fuse_why = w; };
+  if (!cfg_.two_sweep) return no("HIP_TWO_SWEEP = false");
   if (!k.fast || k.ndims != 3) return no("not a 2.5D run kernel");
   if (k.ret_type != "void") return no("a reduction kernel");
   if (k.grids.size() != 2) return no("not exactly two grids");
@@ -1458,7 +1459,7 @@ std::string Translator::SplitInterior(const KernelInfo &k, const std::string &y,
                                       const std::string &z,
                                       const std::string &body,
                                       const std::string &ind) const {
-  const std::string cond = InteriorCond(k, y, z);
+  const std::string cond = cfg_.interior_split ? InteriorCond(k, y, z) : "";
   if (cond.empty()) return body;
   std::ostringstream o;
   o << ind << "auto __pts = [&](auto __interior) {\n"

@@ -30,6 +30,8 @@ struct Config {
   int prefetch = 0;     // 0: chosen from the register estimate (2 or 4)
   bool kernel_error_check = false;  // HIP_KERNEL_ERROR_CHECK
   bool trace = false;               // TRACE_KERNEL
+  bool two_sweep = true;            // HIP_TWO_SWEEP: fuse map pairs
+  bool interior_split = true;       // OPT_LOOP_PEELING: interior wave steps
 };
 
 // One grid parameter of a stencil function.

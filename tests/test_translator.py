@@ -192,3 +192,34 @@ def test_interior_split_only_for_kernels_comparing_y_or_z():
     assert "Flag<true>" not in him
     # x-alignment facts of the VX = 4 form
     assert "__builtin_assume((xs & (VX - 1)) == 0);" in d7
+
+
+def _translate_with_config(conf_text, src):
+    with tempfile.TemporaryDirectory() as d:
+        conf = os.path.join(d, "opt.conf")
+        open(conf, "w").write(conf_text)
+        out = os.path.join(d, "out.hip.cpp")
+        p = subprocess.run([PHYSISC, "--config", conf, src, "-o", out],
+                           capture_output=True, text=True)
+        return p.returncode, (open(out).read() if p.returncode == 0 else ""), \
+            p.stderr
+
+
+def test_reference_configuration_files_are_accepted():
+    # the reference's own configuration files (examples/diffusion-benchmark/
+    # opt.conf, examples/himeno/autotune.conf): optimizer flags, Lua tables
+    conf = """-- optimizer flags of the reference
+OPT_OFFSET_COMP = true
+OPT_LOOP_PEELING = {false, true}
+OPT_REGISTER_BLOCKING = true
+CUDA_BLOCK_SIZE = {{32, 4, 1}, {64, 4, 1}}
+CUDA_KERNEL_ERROR_CHECK = true
+HIP_TWO_SWEEP = false
+"""
+    rc, code, err = _translate_with_config(conf, SOURCES[0])
+    assert rc == 0 and "unknown configuration key" not in err, err
+    assert "Flag<true>" not in code             # OPT_LOOP_PEELING = false
+    assert "__PSStencilRun2_" not in code       # HIP_TWO_SWEEP = false
+    assert "__PSCheckHipError" in code          # CUDA_KERNEL_ERROR_CHECK
+    rc, _, err = _translate_with_config("NOT_A_KEY = 1\n", SOURCES[0])
+    assert rc == 0 and "unknown configuration key NOT_A_KEY" in err
