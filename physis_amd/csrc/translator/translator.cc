@@ -212,6 +212,37 @@ void Translator::FindKernels() {
       FindKernelsIn(*it.func->body);
 }
 
+// A function shaped like a stencil (index parameters, then grid parameters)
+// that emits, but that no map uses: its body only has meaning on the device,
+// so it is left out rather than compiled as host code.
+bool Translator::IsUnusedStencil(const Function &f) const {
+  if (!f.body || kernels_.count(f.name) || device_helpers_.count(f.name))
+    return false;
+  size_t i = 0;
+  while (i < f.params.size() && IsIntType(f.params[i].type)) ++i;
+  if (i == 0 || i > 3 || i == f.params.size() ||
+      !IsGridTypeName(Strip(f.params[i].type), nullptr, nullptr))
+    return false;
+  std::function<bool(const Expr &)> emits = [&](const Expr &e) {
+    const std::string *c = CalleeName(e);
+    if (c && (*c == "PSGridEmit" || *c == "PSGridEmitUtype")) return true;
+    for (const auto &k : e.kids)
+      if (k && emits(*k)) return true;
+    return false;
+  };
+  std::function<bool(const Stmt &)> any = [&](const Stmt &s) {
+    if (s.expr && emits(*s.expr)) return true;
+    if (s.cond && emits(*s.cond)) return true;
+    if (s.init && any(*s.init)) return true;
+    for (const auto &d : s.decls)
+      if (d.init && emits(*d.init)) return true;
+    for (const auto &k : s.kids)
+      if (k && any(*k)) return true;
+    return false;
+  };
+  return any(*f.body);
+}
+
 bool Translator::RegularIndex(const KernelInfo &k, const Expr &e, int axis,
                               int *off) {
   const Expr *x = &e;
@@ -2106,6 +2137,11 @@ std::string Translator::Translate() {
       close_c();
       out << EmitKernel(kernels_[f.name]);
       emitted_kernels.insert(f.name);
+      continue;
+    }
+    if (IsUnusedStencil(f)) {
+      out << "// stencil function " << f.name
+          << ": not used by any map, not emitted\n\n";
       continue;
     }
     std::vector<std::string> ps;

@@ -120,6 +120,7 @@ class Translator {
   void AnalyzeGets(KernelInfo &k, const Stmt &s);
   bool RegularIndex(const KernelInfo &k, const Expr &e, int axis, int *off);
   void Decide(KernelInfo &k);
+  bool IsUnusedStencil(const Function &f) const;
   void AnalyzeFusion(KernelInfo &k);
 
   // device code
