@@ -775,6 +775,21 @@ Program Parser::ParseProgram() {
         }
         Param prm;
         prm.type = ParseBaseType();
+        // an unnamed parameter of a prototype: "int", "char *", "int []"
+        size_t q = pos_;
+        while (t_[q].Is("*") || t_[q].Is("const")) ++q;
+        if (t_[q].Is(",") || t_[q].Is(")") || t_[q].Is("[")) {
+          for (; pos_ < q; ++pos_)
+            prm.type += t_[pos_].Is("*") ? "*" : " const";
+          if (Accept("[")) {
+            while (!Peek().Is("]") && Peek().kind != Tok::kEnd) ++pos_;
+            Expect("]");
+            prm.type += "*";
+          }
+          f->params.push_back(prm);
+          if (!Accept(",")) break;
+          continue;
+        }
         Declarator d = ParseDeclarator(false);
         prm.type += d.prefix.empty() ? "" : " " + d.prefix;
         prm.name = d.name;

@@ -125,3 +125,28 @@ def test_reference_example_on_hip(name):
     assert proc.returncode == 0 and "Error" not in err, (name, err[-2000:])
     if name == "test_staggered_grid":
         assert proc.stdout == _staggered_expected()
+
+
+# The reference's Physis diffusion benchmark (examples/diffusion-benchmark:
+# diffusion3d_physis.c translated by physisc-hip, linked with the benchmark's
+# own driver main.cc / baseline.cc / diffusion3d.cc): its --dump output and
+# its accuracy against the closed-form solution must equal those of the
+# reference's C++ baseline at the same size and count (the fixture's
+# diffusion_baseline records, made from the baseline built in place).
+@pytest.mark.parametrize("size,count", [(32, 4), (24, 10)])
+def test_reference_diffusion_benchmark_on_hip(size, count, tmp_path):
+    exe = os.path.join(EXE_DIR, "diffusion3d_physis.exe")
+    if not os.path.exists(exe):
+        pytest.skip("not built (make -C oracle ref-hip needs the reference "
+                    "sources)")
+    print(f"running the diffusion benchmark {size}^3 x {count}", flush=True)
+    proc = subprocess.run([exe, "--count", str(count), "--size", str(size),
+                           "--dump"], stdout=subprocess.PIPE,
+                          stderr=subprocess.PIPE, timeout=90, cwd=tmp_path)
+    assert proc.returncode == 0, proc.stderr.decode()[-2000:]
+    gold = json.load(open(GOLDEN))["diffusion_baseline"][f"{size}x{count}"]
+    dump = (tmp_path / "diffusion3d_result.physis.out").read_bytes()
+    assert hashlib.sha256(dump).hexdigest() == gold["sha256"]
+    acc = [l for l in proc.stdout.decode().splitlines()
+           if l.startswith("Accuracy")]
+    assert acc and acc[0].split(":")[1].strip() == gold["accuracy"]
