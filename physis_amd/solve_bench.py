@@ -1,7 +1,7 @@
 """Time-to-residual benchmarks of the Poisson extensions on one GPU.
 
     python -m physis_amd.solve_bench precision [--n 512] [--sweeps 20000]
-    python -m physis_amd.solve_bench precision-full [--n 256]
+    python -m physis_amd.solve_bench precision-full [--n 256] [--runs bf16,fp16]
     python -m physis_amd.solve_bench warmstart [--ntrain 256] [--steps 1500]
 
 precision       the 19-point operator at n^3 (coefficients with 1% noise,
@@ -109,24 +109,33 @@ def precision(n: int, sweeps: int, tol: float = 1e-6) -> None:
     _free(grids)
 
 
-def precision_full(n: int, tol: float = 1e-6) -> None:
+def precision_full(n: int, tol: float = 1e-6,
+                   runs=("plain_fp32", "bf16", "fp16"),
+                   max_sweeps: int = 10 ** 7) -> None:
     s = n / 512.0
     coefs = poisson_problem(n, n, n, nbatch=1, noise=0.01, gaussians=(16, 16),
                             sigma=(4.0 * s, 32.0 * s), seed=0)
     shape = (n, n, n)
     grids = _grids(coefs, shape)
     del coefs
-    solver = PoissonSolver(grids, 1, "fp32")
-    p = _grids([np.zeros(n ** 3, np.float32)], shape)[0]
-    st = solver.solve_plain(p, tol=tol, check=500, progress=30)
-    _emit({"bench": "precision-full", "n": n, "storage": "plain_fp32", **st})
-    base = st["seconds"] if st["converged"] else None
-    solver.free()
-    _free([p])
+    base = None
+    if "plain_fp32" in runs:
+        solver = PoissonSolver(grids, 1, "fp32")
+        p = _grids([np.zeros(n ** 3, np.float32)], shape)[0]
+        st = solver.solve_plain(p, tol=tol, check=500, progress=30,
+                                max_sweeps=max_sweeps)
+        _emit({"bench": "precision-full", "n": n, "storage": "plain_fp32",
+               **st})
+        base = st["seconds"] if st["converged"] else None
+        solver.free()
+        _free([p])
     for storage in ("bf16", "fp16"):
+        if storage not in runs:
+            continue
         solver = PoissonSolver(grids, 1, storage)
         p = _grids([np.zeros(n ** 3)], shape, np.float64)[0]
-        st = solver.solve_mixed(p, tol=tol, inner=200, progress=30)
+        st = solver.solve_mixed(p, tol=tol, inner=200, progress=30,
+                                max_sweeps=max_sweeps)
         if base:
             st["speedup_vs_plain_fp32"] = base / st["seconds"]
         _emit({"bench": "precision-full", "n": n,
@@ -235,12 +244,16 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=1500)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--max-sweeps", type=int, default=10 ** 7)
+    ap.add_argument("--runs", default="plain_fp32,bf16,fp16",
+                    help="precision-full: runs to make (comma-separated)")
     a = ap.parse_args(argv)
     ps.PSInit()
     if a.what == "precision":
         precision(a.n or 512, a.sweeps)
     elif a.what == "precision-full":
-        precision_full(a.n or 256)
+        precision_full(a.n or 256, runs=tuple(a.runs.split(",")),
+                       max_sweeps=a.max_sweeps)
     else:
         warmstart(a.ntrain, a.steps, a.batch, a.lr)
     ps.PSFinalize()
