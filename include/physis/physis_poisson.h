@@ -95,8 +95,10 @@ int PSPoissonSolveMixed(__PSPoisson *s, __PSGrid *p, float omega, double tol,
 
 /* Plain FP32 Jacobi: p (FP32 grid) in/out, the FP64 relative residual checked
  * every `check` sweeps (even, >= 2).  Stops when converged, after
- * max_sweeps, or when 20 consecutive checks fail to lower the best residual
- * by 0.1% (FP32 stagnation; st->stagnated = 1).  Returns st->converged. */
+ * max_sweeps, or when 20 consecutive checks spanning at least 20000 sweeps
+ * fail to lower the best residual by 0.1% (FP32 stagnation;
+ * st->stagnated = 1; the mixed solve applies the same rule).  Returns
+ * st->converged. */
 int PSPoissonSolvePlain(__PSPoisson *s, __PSGrid *p, float omega, double tol,
                         int check, long max_sweeps, PSPoissonStats *st);
 

@@ -234,7 +234,7 @@ void oracle_poisson_solve_mixed(int storage, const float *const *cf,
   double *pb = (double *)malloc(sizeof(double) * (size_t)n);
   double *sums = (double *)malloc(sizeof(double) * 3 * (size_t)nbatch);
   double rel, rmax, best;
-  long sweeps = 0;
+  long sweeps = 0, best_at = 0;
   int outer = 0, no_gain = 0, rhs_ready = 0, stagnated = 0;
   oracle_poisson_residual(storage, p, 1, NULL, NULL, 0, 0, NULL, cf, nx, ny,
                           nz, nbatch, sums);
@@ -266,7 +266,8 @@ void oracle_poisson_solve_mixed(int storage, const float *const *cf,
     if (rel < best * (1.0 - 1e-3)) {
       best = rel;
       no_gain = 0;
-    } else if (++no_gain >= 20) {
+      best_at = sweeps;
+    } else if (++no_gain >= 20 && sweeps - best_at >= 20000) {
       stagnated = 1;
       break;
     }

@@ -60,7 +60,7 @@ def _rate(r0: float, r1: float, sweeps: int) -> float:
     return (math.log10(r1) - math.log10(r0)) / sweeps
 
 
-def precision(n: int, sweeps: int, tol: float = 1e-6) -> None:
+def precision(n: int, sweeps: int, tol: float = 1e-6, inner: int = 200) -> None:
     s = n / 512.0
     coefs = poisson_problem(n, n, n, nbatch=1, noise=0.01, gaussians=(16, 16),
                             sigma=(4.0 * s, 32.0 * s), seed=0)
@@ -90,8 +90,9 @@ def precision(n: int, sweeps: int, tol: float = 1e-6) -> None:
     for storage in ("bf16", "fp16"):
         solver = PoissonSolver(grids, 1, storage)
         p = _grids([np.zeros(n ** 3)], shape, np.float64)[0]
-        st = solver.solve_mixed(p, tol=tol, inner=200, max_sweeps=sweeps // 2)
-        st2 = solver.solve_mixed(p, tol=tol, inner=200, max_sweeps=sweeps // 2)
+        st = solver.solve_mixed(p, tol=tol, inner=inner, max_sweeps=sweeps // 2)
+        st2 = solver.solve_mixed(p, tol=tol, inner=inner,
+                                 max_sweeps=sweeps // 2)
         per = st2["seconds"] / st2["sweeps"]
         rate = _rate(st["rel_residual"], st2["rel_residual"], st2["sweeps"])
         need = (math.log10(tol) - math.log10(st2["rel_residual"])) / rate \
@@ -111,10 +112,12 @@ def precision(n: int, sweeps: int, tol: float = 1e-6) -> None:
 
 def precision_full(n: int, tol: float = 1e-6,
                    runs=("plain_fp32", "bf16", "fp16"),
-                   max_sweeps: int = 10 ** 7) -> None:
+                   max_sweeps: int = 10 ** 7, inner: int = 200,
+                   consistent: bool = False) -> None:
     s = n / 512.0
     coefs = poisson_problem(n, n, n, nbatch=1, noise=0.01, gaussians=(16, 16),
-                            sigma=(4.0 * s, 32.0 * s), seed=0)
+                            sigma=(4.0 * s, 32.0 * s), seed=0,
+                            consistent=consistent)
     shape = (n, n, n)
     grids = _grids(coefs, shape)
     del coefs
@@ -125,7 +128,7 @@ def precision_full(n: int, tol: float = 1e-6,
         st = solver.solve_plain(p, tol=tol, check=500, progress=30,
                                 max_sweeps=max_sweeps)
         _emit({"bench": "precision-full", "n": n, "storage": "plain_fp32",
-               **st})
+               "consistent": consistent, **st})
         base = st["seconds"] if st["converged"] else None
         solver.free()
         _free([p])
@@ -134,12 +137,13 @@ def precision_full(n: int, tol: float = 1e-6,
             continue
         solver = PoissonSolver(grids, 1, storage)
         p = _grids([np.zeros(n ** 3)], shape, np.float64)[0]
-        st = solver.solve_mixed(p, tol=tol, inner=200, progress=30,
+        st = solver.solve_mixed(p, tol=tol, inner=inner, progress=30,
                                 max_sweeps=max_sweeps)
+        st["inner_sweeps"] = inner
         if base:
             st["speedup_vs_plain_fp32"] = base / st["seconds"]
         _emit({"bench": "precision-full", "n": n,
-               "storage": f"mixed_{storage}", **st})
+               "storage": f"mixed_{storage}", "consistent": consistent, **st})
         solver.free()
         _free([p])
     _free(grids)
@@ -245,15 +249,21 @@ def main(argv=None) -> int:
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--max-sweeps", type=int, default=10 ** 7)
+    ap.add_argument("--inner", type=int, default=200,
+                    help="inner sweeps per refinement step")
+    ap.add_argument("--consistent", action="store_true",
+                    help="precision-full: a3 = 1 / sum of the face "
+                         "coefficients (b = 0) instead of independent noise")
     ap.add_argument("--runs", default="plain_fp32,bf16,fp16",
                     help="precision-full: runs to make (comma-separated)")
     a = ap.parse_args(argv)
     ps.PSInit()
     if a.what == "precision":
-        precision(a.n or 512, a.sweeps)
+        precision(a.n or 512, a.sweeps, inner=a.inner)
     elif a.what == "precision-full":
         precision_full(a.n or 256, runs=tuple(a.runs.split(",")),
-                       max_sweeps=a.max_sweeps)
+                       max_sweeps=a.max_sweeps, inner=a.inner,
+                       consistent=a.consistent)
     else:
         warmstart(a.ntrain, a.steps, a.batch, a.lr)
     ps.PSFinalize()
