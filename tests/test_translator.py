@@ -223,3 +223,23 @@ HIP_TWO_SWEEP = false
     assert "__PSCheckHipError" in code          # CUDA_KERNEL_ERROR_CHECK
     rc, _, err = _translate_with_config("NOT_A_KEY = 1\n", SOURCES[0])
     assert rc == 0 and "unknown configuration key NOT_A_KEY" in err
+
+
+REF_EXAMPLES = "/root/reference/examples"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_EXAMPLES),
+                    reason="the reference's examples are not present")
+@pytest.mark.parametrize("rel", [
+    "test_double_buffering.c", "test_global_variable.c", "test_set.c",
+    "test_staggered_grid.c", "diffusion-benchmark/diffusion3d_physis.c",
+    "himeno/himenobmtxpa_physis.c", "copy/copy3d.c"])
+def test_reference_examples_translate(rel):
+    """The reference's example and benchmark programs, unchanged."""
+    src = os.path.join(REF_EXAMPLES, rel)
+    with tempfile.TemporaryDirectory() as d:
+        p = subprocess.run([PHYSISC, "-I", os.path.join(ROOT, "include"),
+                            "-I", os.path.dirname(src), src, "-o",
+                            os.path.join(d, "o.hip.cpp")],
+                           capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr
