@@ -32,6 +32,14 @@ inline __PSGrid *NewUserGrid(__PSGridTypeInfo *ti, int num_dims, int d0,
   return __PSGridNew(ti, num_dims, dim, nullptr);
 }
 
+// PSGridGet(g, i...) in host code: one element copied back (synchronous).
+template <typename T>
+inline T GetElem(__PSGrid *g, PSIndex i0, PSIndex i1 = 0, PSIndex i2 = 0) {
+  T v;
+  __PSGridGet(g, &v, i0, i1, i2);
+  return v;
+}
+
 // PSGridSet(g, i..., v) with v converted to the element type.
 template <typename T>
 inline void SetElem(__PSGrid *g, T v, PSIndex i0, PSIndex i1 = 0,

@@ -2056,6 +2056,19 @@ void Translator::RewriteHostExpr(ExprP &e, const Function &f) {
                     ", " + Print(*e->kids.back());
     for (size_t a = 2; a + 1 < e->kids.size(); ++a) s += ", " + Print(*e->kids[a]);
     e = Raw(s + ")");
+  } else if (c == "PSGridGet" || c == "PSGridGetPeriodic") {
+    // a get in host code (reference translator.cc:170-190): one element
+    // read back; periodic gets are rejected there too
+    if (c == "PSGridGetPeriodic")
+      Fail(f.name, "PSGridGetPeriodic is not allowed in host code");
+    std::string el = GridElemOf(*e->kids[1], f);
+    if (el.empty() || el.compare(0, 7, "struct ") == 0 || !IsGridTypeName(
+        "PSGrid3D" + Cap(el), nullptr, nullptr))
+      Fail(f.name, "PSGridGet in host code on a grid of unknown or "
+                   "user-defined type");
+    std::string s = "physis_hip::GetElem<" + el + ">(" + Print(*e->kids[1]);
+    for (size_t a = 2; a < e->kids.size(); ++a) s += ", " + Print(*e->kids[a]);
+    e = Raw(s + ")");
   } else if (c == "PSReduce") {
     std::string el = GridElemOf(*e->kids[3], f);
     if (el.empty()) Fail(f.name, "PSReduce on a grid of unknown type");
@@ -2099,7 +2112,7 @@ std::string Translator::Translate() {
   for (auto &it : prog_.items) {
     if (it.kind != TopItem::kFunction) continue;
     Function &f = *it.func;
-    if (kernels_.count(f.name)) continue;
+    if (kernels_.count(f.name) || device_helpers_.count(f.name)) continue;
     for (auto &p : f.params) {
       std::string t = Strip(p.type);
       if (IsGridTypeName(t, nullptr, nullptr)) local_grids_[f.name][p.name] = t;

@@ -535,3 +535,31 @@ void systest_udt_complex(struct Cplx *out) {
   PSGridFree(g2);
   free(indata);
 }
+
+/* ---- host-side element access (PSGridSet / PSGridGet in host code) */
+
+/* test_set.c-style: elements written one by one with PSGridSet, one read
+ * back with PSGridGet after a map doubled them. */
+static void kdouble(const int x, const int y, const int z, PSGrid3DFloat g,
+                    PSGrid3DFloat g2) {
+  PSGridEmit(g2, PSGridGet(g, x, y, z) * 2.0f);
+}
+
+float systest_host_get(int x, int y, int z) {
+  int i, j, k;
+  float v = 0;
+  PSGrid3DFloat g = PSGrid3DFloatNew(4, 3, 2);
+  PSGrid3DFloat g2 = PSGrid3DFloatNew(4, 3, 2);
+  PSDomain3D d = PSDomain3DNew(0, 4, 0, 3, 0, 2);
+  for (k = 0; k < 2; ++k)
+    for (j = 0; j < 3; ++j)
+      for (i = 0; i < 4; ++i) {
+        PSGridSet(g, i, j, k, v);
+        v += 1;
+      }
+  PSStencilRun(PSStencilMap(kdouble, d, g, g2));
+  v = PSGridGet(g2, x, y, z);
+  PSGridFree(g);
+  PSGridFree(g2);
+  return v;
+}

@@ -91,3 +91,12 @@ def test_user_type_whole_point_periodic(prog):
     prog.systest_udt_complex(out.ctypes.data_as(C.c_void_p))
     assert sha256(_dump_rows(out)) == \
         GOLD["test_user-defined-type-7-pt-periodic-complex"]["sha256"]
+
+
+def test_host_set_and_get(prog):
+    # PSGridSet of i at (x, y, z) in x-fastest order, a map doubling it,
+    # PSGridGet of single elements in host code
+    prog.systest_host_get.restype = C.c_float
+    prog.systest_host_get.argtypes = [C.c_int] * 3
+    for x, y, z in ((0, 0, 0), (3, 2, 1), (1, 2, 0)):
+        assert prog.systest_host_get(x, y, z) == 2.0 * (x + 4 * y + 12 * z)

@@ -243,3 +243,14 @@ def test_reference_examples_translate(rel):
                             os.path.join(d, "o.hip.cpp")],
                            capture_output=True, text=True)
     assert p.returncode == 0, p.stderr
+
+
+def test_host_get_reads_one_element():
+    src = """#include "physis/physis.h"
+float first(PSGrid3DFloat g) { return PSGridGet(g, 0, 1, 2); }
+"""
+    rc, code, err = translate(None, src)
+    assert rc == 0, err
+    assert "physis_hip::GetElem<float>(g, 0, 1, 2)" in code
+    rc, _, err = translate(None, src.replace("PSGridGet(", "PSGridGetPeriodic("))
+    assert rc != 0 and "not allowed in host code" in err
