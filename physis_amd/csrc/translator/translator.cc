@@ -1887,13 +1887,14 @@ std::string Translator::EmitRunFunction(const RunSite &r) {
     // pairs leave them
     const KernelInfo &k = kernels_.at(r.kernels[0]);
     const GridParam &gi = k.grids[k.fuse_in], &go = k.grids[k.fuse_out];
+    // Each pass is one pair.  Even counts ping-pong a -> t -> a ...; odd
+    // counts (>= 3) send the first pass to b instead, a -> b -> t -> a ...
+    // (b is free until the last pass writes it).  Either way the last pass
+    // reads t and writes a, and its intermediate sweep to b.
     o << "  int i0 = 0;\n"
       << "  if (iter >= 2 && __PSFusable2_" << k.name << "(s0, s1)) {\n"
-      << "    if (iter % 2) {\n      __PSRun_" << k.name << "(s0, stream, NULL);\n"
-      << "      __PSRun_" << k.name << "(s1, stream, NULL);\n    }\n"
       << "    physis_hip::CheckGrid3D(s0." << gi.name << ", " << gi.ps_type << ");\n"
       << "    physis_hip::CheckGrid3D(s0." << go.name << ", " << go.ps_type << ");\n"
-      << "    const int nf = iter - iter % 2;\n"
       << "    " << gi.dev_view << " va = physis_hip::DevView<" << gi.dev_view
       << ">(s0." << gi.name << ");\n"
       << "    " << gi.dev_view << " vb = physis_hip::DevView<" << gi.dev_view
@@ -1902,9 +1903,13 @@ std::string Translator::EmitRunFunction(const RunSite &r) {
       << "    vt.p = (" << gi.elem << " *)__PSHipScratch(8, (size_t)s0." << gi.name
       << "->num_elms * sizeof(" << gi.elem << "));\n"
       << "    " << gi.dev_view << " src = va;\n"
-      << "    for (int j = 0; j < nf; ++j) {\n"
-      << "      const " << gi.dev_view << " dst = (j % 2 == 0) ? vt : va;\n"
-      << "      __PSLaunch2_" << k.name << "(s0, src, dst, j == nf - 1 ? vb.p : NULL, stream);\n"
+      << "    int j = 0;\n"
+      << "    if (iter % 2) {\n"
+      << "      __PSLaunch2_" << k.name << "(s0, va, vb, NULL, stream);\n"
+      << "      src = vb;\n      j = 1;\n    }\n"
+      << "    for (; j < iter; ++j) {\n"
+      << "      const " << gi.dev_view << " dst = ((iter - 1 - j) % 2 == 0) ? va : vt;\n"
+      << "      __PSLaunch2_" << k.name << "(s0, src, dst, j == iter - 1 ? vb.p : NULL, stream);\n"
       << "      src = dst;\n    }\n"
       << "    i0 = iter;\n  }\n";
   }
