@@ -3,8 +3,8 @@
  * 19-point operator of physis_poisson.h from its right-hand side, trained on
  * the GPU by hand-written forward and backward kernels (3x3x3 convolutions
  * on BF16 MFMA with FP32 accumulation).  Jacobi then runs from the
- * prediction to the requested residual; it is both the fallback and the
- * judge of the network's error.  No counterpart in the reference.
+ * prediction to the requested residual, so it both falls back for and
+ * measures the network's error.  No counterpart in the reference.
  *
  * Network (channels-last BF16 activations, FP32 master weights):
  *   in  (8 channels: rhs, bnd, 6 zero)
