@@ -6,7 +6,8 @@ Run under the profiler, e.g.
 It issues, in order: a device-to-device copy of one 512 MiB grid (the
 runtime's plain copy kernel, whose FETCH_SIZE / WRITE_SIZE calibrate the
 counters' units), at 512^3 four two-sweep launches of the 7-point clamped
-kernel, five single sweeps of it, 10 periodic sweeps, 3 PSReduce SUMs at 512^3, 10 Himeno sweeps at 512x256x256, and
+kernel, five single sweeps of it, 10 periodic sweeps (five two-sweep
+launches), 3 PSReduce SUMs at 512^3, 10 Himeno sweeps at 512x256x256, and
 the precision tier at 512^3: 10 BF16 inner sweeps and 3 fused FP64
 update + residual passes.
 Then `python3 -m physis_amd.profile_summary OUT_fetch OUT_write` writes the

@@ -27,8 +27,6 @@ KERNELS = [  # (label, name fragment, algorithmic bytes per launch)
      8 * 512 ** 3),
     ("7-pt periodic two-sweep kernel, 512^3 (2 sweeps per launch)",
      "__PSStencilRun2_diffusion7_periodic_kernel<", 8 * 512 ** 3),
-    ("7-pt periodic run kernel, 512^3",
-     "__PSStencilRun_diffusion7_periodic_kernel<", 8 * 512 ** 3),
     ("19-pt Himeno run kernel, 512x256x256", "__PSStencilRun_jacobi_kernel<",
      56 * 512 * 256 * 256),
     ("PSReduce SUM stage 1, 512^3", "ReduceStage1<float, 2>", 4 * 512 ** 3),
