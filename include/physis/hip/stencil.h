@@ -106,6 +106,10 @@ constexpr int kBufDword3 = 0x00020000;  // raw 32-bit dwords, gfx9 family
 constexpr int kDrop = INT32_MIN;
 // Non-temporal (streaming) cache policy bit of buffer stores on gfx950.
 constexpr int kStreamStore = 2;
+// The same bit on buffer loads: data read once (centre-only coefficient
+// grids) is marked for early eviction, which leaves the L2 to the staged
+// planes whose halo rows neighbouring blocks read again.
+constexpr int kStreamLoad = 2;
 
 template <typename T>
 struct BufWin {
@@ -129,33 +133,34 @@ struct BufWin {
     return off == kDrop ? kDrop : (off - start) * (int)sizeof(T);
   }
 
-  template <int N>
+  // AUX is the cache-policy operand of the buffer load (kStreamLoad or 0).
+  template <int N, int AUX = 0>
   __device__ __forceinline__ void Load(int off, T (&out)[N]) const {
     constexpr int kBytes = (int)sizeof(T) * N;
     const int b = Bytes(off);
     if constexpr (kBytes == 32) {
-      auto v0 = __builtin_amdgcn_raw_buffer_load_b128(rsrc, b, 0, 0);
+      auto v0 = __builtin_amdgcn_raw_buffer_load_b128(rsrc, b, 0, AUX);
       auto v1 = __builtin_amdgcn_raw_buffer_load_b128(
-          rsrc, b == kDrop ? kDrop : b + 16, 0, 0);
+          rsrc, b == kDrop ? kDrop : b + 16, 0, AUX);
       __builtin_memcpy(out, &v0, 16);
       __builtin_memcpy((char *)out + 16, &v1, 16);
     } else if constexpr (kBytes == 16) {
-      auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, b, 0, 0);
+      auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, b, 0, AUX);
       __builtin_memcpy(out, &v, 16);
     } else if constexpr (kBytes == 8) {
-      auto v = __builtin_amdgcn_raw_buffer_load_b64(rsrc, b, 0, 0);
+      auto v = __builtin_amdgcn_raw_buffer_load_b64(rsrc, b, 0, AUX);
       __builtin_memcpy(out, &v, 8);
     } else if constexpr (kBytes == 4) {
-      auto v = __builtin_amdgcn_raw_buffer_load_b32(rsrc, b, 0, 0);
+      auto v = __builtin_amdgcn_raw_buffer_load_b32(rsrc, b, 0, AUX);
       __builtin_memcpy(out, &v, 4);
     } else if constexpr (kBytes == 2) {
-      auto v = __builtin_amdgcn_raw_buffer_load_b16(rsrc, b, 0, 0);
+      auto v = __builtin_amdgcn_raw_buffer_load_b16(rsrc, b, 0, AUX);
       __builtin_memcpy(out, &v, 2);
     } else {
 #pragma unroll
       for (int i = 0; i < N; ++i) {
         T one[1];
-        Load<1>(off == kDrop ? kDrop : off + i, one);
+        Load<1, AUX>(off == kDrop ? kDrop : off + i, one);
         out[i] = one[0];
       }
     }

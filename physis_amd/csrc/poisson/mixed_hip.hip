@@ -10,7 +10,9 @@
 //                      arithmetic in the operand order of the Himeno
 //                      jacobi_kernel (reference examples/himeno/
 //                      himenobmtxpa_physis.c:340-359).  Bytes per point:
-//                      14 * sizeof(M).
+//                      14 * sizeof(M).  The 12 centre-only fields are read
+//                      by the compute waves only, non-temporally when they
+//                      exceed the last-level cache.
 //   ResidualKernel     FP64 residual r = (s0 - p / a3) bnd from FP32
 //                      coefficients, optionally fused with the refinement
 //                      update p' = p + E / sigma (written to a second FP64
@@ -120,7 +122,10 @@ __device__ __forceinline__ ZRange BlockZ(const Plan &pl, int bz) {
 }
 
 // ------------------------------------------------------------- Jacobi sweep
-template <typename M, int VX>
+// LAUX: cache policy of the coefficient loads (physis_hip::kStreamLoad when
+// the fields exceed the last-level cache, else 0: re-read every sweep, they
+// then stay cached).
+template <typename M, int VX, int LAUX>
 __global__ __launch_bounds__(kWave *kBY) void SweepKernel(
     const M *__restrict__ pin, M *__restrict__ pout, Fields<M> f, int nx,
     int ny, int nz, float omega, Plan pl) {
@@ -137,8 +142,11 @@ __global__ __launch_bounds__(kWave *kBY) void SweepKernel(
   const int yrow = y0 - 1 + wave;
   const bool computes = wave >= 1 && wave < kBY - 1 && yrow < ny;
   const int xs = x0 + lane * VX;
-  // centre-only grids: halo waves repeat a compute row's addresses
-  const int drow = wave < 1 ? y0 : (wave >= kBY - 1 ? y0 + TY - 1 : yrow);
+  // centre-only grids: halo waves issue the same loads at an out-of-range
+  // offset, which the buffer range check drops (no memory access), so
+  // every wave keeps the same count of loads in flight
+  const bool dl = wave >= 1 && wave < kBY - 1;
+  const int drow = dl ? yrow : y0;
   const int nxy = nx * ny, n = nxy * nz;
   const int dim[3] = {nx, ny, nz};
   const BufWin<M> pw =
@@ -161,7 +169,8 @@ __global__ __launch_bounds__(kWave *kBY) void SweepKernel(
 #pragma unroll
   for (int g = 0; g < kNumF; ++g) {
     fw[g] = BufWin<M>::Make(f.g[g], n, (long)zb * nxy, (long)ze * nxy);
-    fw[g].template Load<VX>(doff + zb * nxy, fv[g][0]);
+    fw[g].template Load<VX, LAUX>(
+        dl ? doff + zb * nxy : physis_hip::kDrop, fv[g][0]);
   }
   __syncthreads();
   for (int zc = zb; zc < ze; zc += kPD) {
@@ -170,7 +179,8 @@ __global__ __launch_bounds__(kWave *kBY) void SweepKernel(
       const int z = zc + k, cur = k & 1, nxt = cur ^ 1;
 #pragma unroll
       for (int g = 0; g < kNumF; ++g)
-        fw[g].template Load<VX>(doff + (z + 1) * nxy, fv[g][nxt]);
+        fw[g].template Load<VX, LAUX>(
+            dl ? doff + (z + 1) * nxy : physis_hip::kDrop, fv[g][nxt]);
       if (computes && z < ze) {
         Rows19<float, VX> r;
         r.template Read<S>(lds, base, wave, lane);
@@ -408,16 +418,31 @@ Fields<float> MasterFields(const __PSPoisson *s) {
   return f;
 }
 
+// MI355X's last-level cache (AMD Infinity Cache): 256 MiB.
+constexpr size_t kLastLevelCacheBytes = size_t(256) << 20;
+
+template <typename M, int VX>
+void LaunchSweepVX(const __PSPoisson *s, const M *in, M *out, Fields<M> f,
+                   float omega) {
+  const dim3 grid(s->sweep_plan.blocks), block(kWave * kBY);
+  const size_t field_bytes =
+      (size_t)kNumF * s->nx * s->ny * s->nz * sizeof(M);
+  if (field_bytes > kLastLevelCacheBytes)
+    hipLaunchKernelGGL((SweepKernel<M, VX, physis_hip::kStreamLoad>), grid,
+                       block, 0, Stream(), in, out, f, s->nx, s->ny, s->nz,
+                       omega, s->sweep_plan);
+  else
+    hipLaunchKernelGGL((SweepKernel<M, VX, 0>), grid, block, 0, Stream(), in,
+                       out, f, s->nx, s->ny, s->nz, omega, s->sweep_plan);
+}
+
 template <typename M>
 void LaunchSweep(const __PSPoisson *s, const M *in, M *out, Fields<M> f,
                  float omega) {
-  const dim3 grid(s->sweep_plan.blocks), block(kWave * kBY);
   if (s->sweep_vx == 4)
-    hipLaunchKernelGGL((SweepKernel<M, 4>), grid, block, 0, Stream(), in, out,
-                       f, s->nx, s->ny, s->nz, omega, s->sweep_plan);
+    LaunchSweepVX<M, 4>(s, in, out, f, omega);
   else
-    hipLaunchKernelGGL((SweepKernel<M, 2>), grid, block, 0, Stream(), in, out,
-                       f, s->nx, s->ny, s->nz, omega, s->sweep_plan);
+    LaunchSweepVX<M, 2>(s, in, out, f, omega);
   CheckLaunch();
 }
 
@@ -623,18 +648,18 @@ __PSPoisson *PSPoissonNew(__PSGrid *const coef[12], int nbatch, int storage) {
   switch (storage) {
     case PS_STORAGE_BF16:
       per_cu_sweep = physis_hip::BlocksPerCU(
-          s->sweep_vx == 4 ? (const void *)SweepKernel<bf16_t, 4>
-                           : (const void *)SweepKernel<bf16_t, 2>, kWave * kBY);
+          s->sweep_vx == 4 ? (const void *)SweepKernel<bf16_t, 4, 0>
+                           : (const void *)SweepKernel<bf16_t, 2, 0>, kWave * kBY);
       break;
     case PS_STORAGE_FP16:
       per_cu_sweep = physis_hip::BlocksPerCU(
-          s->sweep_vx == 4 ? (const void *)SweepKernel<fp16_t, 4>
-                           : (const void *)SweepKernel<fp16_t, 2>, kWave * kBY);
+          s->sweep_vx == 4 ? (const void *)SweepKernel<fp16_t, 4, 0>
+                           : (const void *)SweepKernel<fp16_t, 2, 0>, kWave * kBY);
       break;
     default:
       per_cu_sweep = physis_hip::BlocksPerCU(
-          s->sweep_vx == 4 ? (const void *)SweepKernel<float, 4>
-                           : (const void *)SweepKernel<float, 2>, kWave * kBY);
+          s->sweep_vx == 4 ? (const void *)SweepKernel<float, 4, 0>
+                           : (const void *)SweepKernel<float, 2, 0>, kWave * kBY);
   }
   per_cu_res = physis_hip::BlocksPerCU(
       (const void *)ResidualKernel<double, bf16_t, 2, true, true>, kWave * kBY);

@@ -1201,7 +1201,8 @@ std::string Translator::EmitFastRunKernel(const KernelInfo &k) {
         << G << "_nxy * " << G << ".dim[2], (long)zb * " << G
         << "_nxy, (long)ze * " << G << "_nxy);\n";
       o << "    " << g.elem << " " << G << "_dv[2][VX];\n";
-      o << "    " << G << "_w.template Load<VX>(xs + drow * " << G
+      o << "    " << G
+        << "_w.template Load<VX, physis_hip::kStreamLoad>(xs + drow * " << G
         << "_nx + zb * " << G << "_nxy, " << G << "_dv[0]);\n";
     }
     if (g.emitted) {
@@ -1255,7 +1256,8 @@ std::string Translator::EmitFastRunKernel(const KernelInfo &k) {
         << ";\n";
   for (auto &g : k.grids)
     if (g.mode == GridParam::kDirect)
-      o << "        " << g.name << "_w.template Load<VX>(xs + drow * " << g.name
+      o << "        " << g.name
+        << "_w.template Load<VX, physis_hip::kStreamLoad>(xs + drow * " << g.name
         << "_nx + (z + 1) * " << g.name << "_nxy, " << g.name << "_dv[nxt]);\n";
   o << "        if (computes && z < ze) {\n";
   for (auto &g : k.grids) {
