@@ -4,7 +4,8 @@
 // context.cc:439-490); ROSE is not available, so physisc-hip carries its own
 // small tree: enough C to hold stencil functions (expressions, declarations,
 // if/for/while/do/switch/return) and to find and rewrite the PS* intrinsics
-// in host code.  Host code outside intrinsics is kept as token runs.
+// in host code.  Host code outside intrinsics is kept as token runs, and so
+// are top-level C++ declarations (class, using, namespace, template).
 #ifndef PHYSIS_AMD_TRANSLATOR_AST_H_
 #define PHYSIS_AMD_TRANSLATOR_AST_H_
 
@@ -67,6 +68,7 @@ struct Declarator {
   std::string prefix;  // pointer stars / qualifiers before the name
   std::string suffix;  // array bounds after the name
   ExprP init;
+  bool ctor_init = false;  // C++ "T v(args)": init holds the argument list
 };
 
 enum class SK {
@@ -115,6 +117,8 @@ struct TopItem {
   std::vector<Token> tokens;   // kTokens: verbatim declaration
   std::string include_line;    // kInclude
   std::string user_grid;       // kTokens of DeclareGrid<n>D(Name, type): Name
+  bool cxx = false;            // kTokens of a C++-only declaration (class,
+                               // using, namespace, template): no C linkage
 };
 
 struct Program {

@@ -249,9 +249,11 @@ std::string Parser::ParseStructBody(const std::string &tag) {
 
 Declarator Parser::ParseDeclarator(bool allow_init) {
   Declarator d;
-  while (Peek().Is("*") || Peek().Is("const") || Peek().Is("restrict") ||
-         Peek().Is("__restrict__")) {
-    d.prefix += Peek().text == "*" ? "*" : " " + Peek().text + " ";
+  while (Peek().Is("*") || Peek().Is("&") || Peek().Is("const") ||
+         Peek().Is("restrict") || Peek().Is("__restrict__")) {
+    d.prefix += (Peek().text == "*" || Peek().text == "&")
+                    ? Peek().text
+                    : " " + Peek().text + " ";
     ++pos_;
   }
   if (Peek().kind != Tok::kIdent) Fail("expected a name");
@@ -271,6 +273,13 @@ Declarator Parser::ParseDeclarator(bool allow_init) {
       d.init = ParseInitList();
     else
       d.init = ParseAssign();
+  } else if (allow_init && d.suffix.empty() && Peek().Is("(") &&
+             !Peek(1).Is(")")) {
+    // C++ direct initialisation "T v(a, b)"
+    ++pos_;
+    d.init = ParseExpr();
+    d.ctor_init = true;
+    Expect(")");
   }
   return d;
 }
@@ -696,6 +705,35 @@ Program Parser::ParseProgram() {
       prog.items.push_back(std::move(it));
       continue;
     }
+    if (t.Is("class") || t.Is("using") || t.Is("namespace") ||
+        t.Is("template")) {
+      // C++ declarations outside the Physis subset are kept as written:
+      // up to the ';' that ends them at brace depth 0, or to the closing
+      // brace of a namespace body.  A class name becomes a type name.
+      const bool ns = t.Is("namespace");
+      if (t.Is("class") && Peek(1).kind == Tok::kIdent)
+        typedefs_.insert(Peek(1).text);
+      int depth = 0;
+      while (true) {
+        if (Peek().kind == Tok::kEnd) Fail("unterminated declaration");
+        if (Peek().Is("{")) ++depth;
+        if (Peek().Is("}") && --depth == 0 && ns) {
+          ++pos_;
+          break;
+        }
+        if (depth == 0 && Peek().Is(";")) {
+          ++pos_;
+          break;
+        }
+        ++pos_;
+      }
+      TopItem it;
+      it.kind = TopItem::kTokens;
+      it.cxx = true;
+      it.tokens.assign(t_.begin() + start, t_.begin() + pos_);
+      prog.items.push_back(std::move(it));
+      continue;
+    }
     if (t.Is("typedef")) {
       // typedef ... name ;  (register the last identifier at depth 0)
       int depth = 0;
@@ -878,7 +916,10 @@ std::string PrintDecl(const Stmt &s) {
   for (size_t i = 0; i < s.decls.size(); ++i) {
     const Declarator &d = s.decls[i];
     out += (i ? ", " : "") + d.prefix + d.name + d.suffix;
-    if (d.init) out += " = " + Print(*d.init);
+    if (d.init && d.ctor_init)
+      out += "(" + Print(*d.init) + ")";
+    else if (d.init)
+      out += " = " + Print(*d.init);
   }
   return out + ";";
 }

@@ -2140,7 +2140,10 @@ std::string Translator::Translate() {
   for (auto &it : prog_.items) {
     if (it.kind == TopItem::kInclude) continue;
     if (it.kind == TopItem::kTokens) {
-      open_c();
+      if (it.cxx)
+        close_c();
+      else
+        open_c();
       out << PrintTokens(it.tokens) << "\n\n";
       if (!it.user_grid.empty() && !user_types.count(it.user_grid)) {
         user_types.insert(it.user_grid);

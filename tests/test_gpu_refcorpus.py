@@ -25,8 +25,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 EXE_DIR = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "hip")
 GOLD = json.load(open(GOLDEN))["system_tests"]
 
-# Every test program of the reference's system-test corpus except the C++ one
-# (test_cplusplus.cc).
+# Every test program of the reference's system-test corpus (test_cplusplus is
+# the one C++ source, test_cplusplus.cc).
 CORPUS = [
     "test_01", "test_02", "test_03", "test_08", "test_09", "test_10",
     "test_15", "test_16", "test_27-pt", "test_27-pt-periodic",
@@ -53,7 +53,7 @@ CORPUS = [
     "test_user-defined-type-multi-members",
     "test_user-defined-type-transpose", "test_user-defined-type1",
     "test_user-defined-type2", "test_user-defined-type3",
-    "test_user-defined-type5",
+    "test_user-defined-type5", "test_cplusplus",
 ]
 
 

@@ -116,6 +116,8 @@ def test_reference_corpus_translates():
     inc = os.path.join(ROOT, "include")
     for name in CORPUS:
         src = os.path.join(REF_CASES, name + ".c")
+        if not os.path.exists(src):
+            src += "c"  # the C++ test
         with tempfile.TemporaryDirectory() as d:
             p = subprocess.run([PHYSISC, "-I", inc, src, "-o",
                                 os.path.join(d, "o.hip.cpp")],
@@ -254,3 +256,53 @@ float first(PSGrid3DFloat g) { return PSGridGet(g, 0, 1, 2); }
     assert "physis_hip::GetElem<float>(g, 0, 1, 2)" in code
     rc, _, err = translate(None, src.replace("PSGridGet(", "PSGridGetPeriodic("))
     assert rc != 0 and "not allowed in host code" in err
+
+
+CXX_PROG = """#include <iostream>
+#include "physis/physis.h"
+using namespace std;
+class Scale {
+ public:
+  Scale(int s) { s_ = s; }
+  int operator()() { return s_; }
+ private:
+  int s_;
+};
+void copy(const int x, const int y, const int z, PSGrid3DFloat a,
+          PSGrid3DFloat b) {
+  PSGridEmit(b, PSGridGet(a, x, y, z));
+}
+void run(Scale &s, PSGrid3DFloat a, PSGrid3DFloat b) {
+  float k = (float)s();
+  PSDomain3D d = PSDomain3DNew(0, 4, 0, 4, 0, 4);
+  PSStencilRun(PSStencilMap(copy, d, a, b));
+}
+int main(int argc, char *argv[]) {
+  PSInit(&argc, &argv, 3, 4, 4, 4);
+  Scale s(2);
+  return s() - 2;
+}
+"""
+
+
+def test_cplusplus_declarations_are_kept():
+    """C++ declarations (a class, a using-directive) pass through unchanged
+    and outside the C-linkage block; reference parameters and direct
+    initialisation "T v(args)" are parsed (the reference's test_cplusplus.cc
+    uses all of them)."""
+    rc, code, err = translate(None, CXX_PROG)
+    assert rc == 0, err
+    cls = code.index("class Scale")
+    assert code.index("using namespace std;") < cls
+    assert 'extern "C" {' not in code[:cls]
+    assert "void run(Scale & s, PSGrid3DFloat a, PSGrid3DFloat b)" in code
+    assert "Scale s(2);" in code and "(float)s()" in code
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "cxx.hip.cpp")
+        open(src, "w").write(code)
+        p = subprocess.run(["/opt/rocm/bin/hipcc", "-x", "hip",
+                            "--offload-arch=gfx950", "-std=c++17",
+                            "-fsyntax-only", "-w",
+                            "-I" + os.path.join(ROOT, "include"), src],
+                           capture_output=True, text=True)
+        assert p.returncode == 0, p.stderr[-3000:]
