@@ -149,8 +149,71 @@ def ktrace_md(kt_dir: str, out_md: str, command: str) -> None:
     print("\n".join(lines))
 
 
+# U-Net convolution shapes (input, output channels) -> pyramid level, and
+# positions per launch at level 1..3 for batch 8 of 128^3 problems
+UNET_LEVEL = {(8, 8): 1, (24, 8): 1, (8, 24): 1, (8, 16): 2, (16, 8): 2,
+              (16, 16): 2, (48, 16): 2, (16, 48): 2, (16, 32): 3,
+              (32, 16): 3, (32, 32): 3}
+UNET_POS = {1: 8 * 128 ** 3, 2: 8 * 32 ** 3, 3: 8 * 8 ** 3}
+MFMA_BF16_PEAK = 2.5e15  # dense BF16 FLOP/s
+HBM_PEAK = 8e12          # B/s
+
+
+def unet_md(kt_dir: str, out_md: str, command: str) -> None:
+    """Convolution kernels of physis_amd.profile_unet against their
+    roofline: useful flops 2*27*Cin*Cout per position, algorithmic bytes
+    2*(Cin + Cout) per position (BF16 input read once, output or the
+    output gradient read/written once), time bound max(flops / MFMA peak,
+    bytes / HBM peak)."""
+    import re
+    rows = []
+    for f in glob.glob(os.path.join(kt_dir, "*_kernel_stats.csv")):
+        for r in csv.DictReader(open(f)):
+            m = re.search(r"ConvKernel<(\d+), (\d+), (true|false)>", r["Name"])
+            w = re.search(r"WgradKernel<(\d+), (\d+)>", r["Name"])
+            if m:
+                cin, cout = int(m.group(1)), int(m.group(2))
+                what = "forward" if m.group(3) == "true" else "backward data"
+            elif w:
+                cin, cout = int(w.group(1)), int(w.group(2))
+                what = "backward filter"
+            else:
+                continue
+            lvl = UNET_LEVEL[(cin, cout)]
+            pos = UNET_POS[lvl]
+            flops = 2 * 27 * cin * cout * pos
+            byts = 2 * (cin + cout) * pos
+            t = float(r["AverageNs"]) * 1e-9
+            tb = max(flops / MFMA_BF16_PEAK, byts / HBM_PEAK)
+            rows.append((what, cin, cout, lvl, int(r["Calls"]), t, flops,
+                         byts, tb, float(r["Percentage"])))
+    lines = ["# U-Net convolution kernels against their roofline (MI355X)", "",
+             "```", command, "```", "",
+             "Batch 8 of 128^3 problems: level 1 = 8 x 128^3, level 2 = "
+             "8 x 32^3, level 3 = 8 x 8^3 positions.  Useful flops "
+             "2*27*Cin*Cout per position (MFMA padding not counted); "
+             "algorithmic bytes 2*(Cin+Cout) per position; roofline time = "
+             "max(flops / 2.5 PFLOP/s, bytes / 8 TB/s).", "",
+             "| kernel | Cin -> Cout | level | calls | avg us | TFLOP/s | "
+             "% of MFMA peak | GB/s | % of HBM peak | roofline us | "
+             "% of roofline | % of profile time |",
+             "|---|---|---|---|---|---|---|---|---|---|---|---|"]
+    for (what, cin, cout, lvl, calls, t, fl, by, tb,
+         pct) in sorted(rows, key=lambda r: (-r[9])):
+        lines.append(
+            f"| {what} | {cin} -> {cout} | {lvl} | {calls} | {t*1e6:.1f} | "
+            f"{fl/t/1e12:.1f} | {fl/t/MFMA_BF16_PEAK*100:.2f} | "
+            f"{by/t/1e9:.0f} | {by/t/HBM_PEAK*100:.1f} | {tb*1e6:.1f} | "
+            f"{tb/t*100:.1f} | {pct:.1f} |")
+    lines.append("")
+    open(out_md, "w").write("\n".join(lines))
+    print("\n".join(lines))
+
+
 if __name__ == "__main__":
-    if len(sys.argv) == 5 and sys.argv[1] == "--ktrace":
+    if len(sys.argv) == 5 and sys.argv[1] == "--unet":
+        unet_md(*sys.argv[2:])
+    elif len(sys.argv) == 5 and sys.argv[1] == "--ktrace":
         ktrace_md(*sys.argv[2:])
     elif len(sys.argv) == 5:
         main(*sys.argv[1:])
