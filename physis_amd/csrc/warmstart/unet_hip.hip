@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void ConvKernel(
     int W) {
   constexpr int K = 27 * CIN, KB = (K + 31) / 32, KP = KB * 32;
   constexpr int NT = (COUT + 15) / 16;
-  constexpr bool kPre = KB * NT <= 24;
+  constexpr bool kPre = KB * NT <= 14;  // more costs occupancy (24->8: 2)
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int g = lane >> 4, r = lane & 15;
   const int yy = blockIdx.x * 4 + wv;
@@ -238,6 +238,131 @@ __global__ __launch_bounds__(256) void WgradKernel(
     for (int i = 0; i < kLanes; ++i) t += red[threadIdx.x + COUT * i];
     bpart[(long)blockIdx.x * COUT + threadIdx.x] = t;
   }
+}
+
+// Filter gradient partials through LDS, for rows whose width is a multiple
+// of 32.  Block b covers positions [b*chunk, (b+1)*chunk) (a multiple of 32)
+// and every column of k; each step takes 32 positions of one row:
+//   - dz of the 32 positions is staged transposed, dzT[co][p], so the A
+//     operand of a lane (channel co, positions 8g .. 8g+7) is one 16-byte
+//     LDS read;
+//   - x of the 3 x 3 neighbouring rows (z-1..z+1, y-1..y+1) at x0-1 ..
+//     x0+32 is staged transposed, xT[row][ci][x], zero outside the grid, so
+//     the B operand of column (tap, ci) is 8 consecutive LDS elements;
+//   - wave w accumulates column tiles w, w+4, w+8, ... against the shared
+//     A operands.
+// Both operand tiles come from 16-byte global loads of whole channel
+// vectors, in place of the 2-byte gathers of WgradKernel.  Partials and
+// bias partials have WgradKernel's layout; the bias is summed in position
+// order by threads 0 .. COUT-1.
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void WgradLdsKernel(
+    const u16 *__restrict__ x, const u16 *__restrict__ dz,
+    float *__restrict__ part, float *__restrict__ bpart, int P, int D, int H,
+    int W, int chunk) {
+  constexpr int K = 27 * CIN, NTN = (K + 15) / 16, MT = (COUT + 15) / 16;
+  constexpr int TPW = (NTN + 3) / 4;  // column tiles per wave
+  constexpr int XR = 34;              // staged x: x0-1 .. x0+32
+  __shared__ u16 xT[9 * CIN * XR];
+  __shared__ __attribute__((aligned(16))) u16 dzT[COUT * 32];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const int p0 = blockIdx.x * chunk;
+  const int p1 = p0 + chunk < P ? p0 + chunk : P;
+  // LDS offset of each tile's column for this lane (-1: padding column)
+  int lofs[TPW];
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) {
+    const int col = (wv + 4 * i) * 16 + r;
+    if (col < K) {
+      const int tap = col / CIN, ci = col - tap * CIN;
+      lofs[i] = ((tap / 3) * CIN + ci) * XR + tap % 3;
+    } else {
+      lofs[i] = -1;
+    }
+  }
+  f32x4 acc[TPW][MT];
+#pragma unroll
+  for (int i = 0; i < TPW; ++i)
+#pragma unroll
+    for (int m = 0; m < MT; ++m) acc[i][m] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float bsum = 0.0f;
+  for (int s = p0; s < p1; s += 32) {
+    int q = s;
+    const int x0 = q % W;
+    q /= W;
+    const int y = q % H;
+    q /= H;
+    const int z = q % D;
+    const int n = q / D;
+    for (int v = threadIdx.x; v < 9 * XR * (CIN / 8); v += 256) {
+      const int c8 = v % (CIN / 8), t = v / (CIN / 8);
+      const int xp = t % XR, row = t / XR;
+      const int zz = z + row / 3 - 1, yy = y + row % 3 - 1, xx = x0 - 1 + xp;
+      uint4 u = make_uint4(0, 0, 0, 0);
+      if (zz >= 0 && zz < D && yy >= 0 && yy < H && xx >= 0 && xx < W)
+        u = *reinterpret_cast<const uint4 *>(
+            x + ((((long)n * D + zz) * H + yy) * W + xx) * CIN + c8 * 8);
+      const unsigned w4[4] = {u.x, u.y, u.z, u.w};
+      u16 *dst = xT + (row * CIN + c8 * 8) * XR + xp;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        dst[(2 * j) * XR] = (u16)(w4[j] & 0xffffu);
+        dst[(2 * j + 1) * XR] = (u16)(w4[j] >> 16);
+      }
+    }
+    for (int v = threadIdx.x; v < 32 * (COUT / 8); v += 256) {
+      const int c8 = v % (COUT / 8), pp = v / (COUT / 8);
+      const uint4 u = *reinterpret_cast<const uint4 *>(
+          dz + (long)(s + pp) * COUT + c8 * 8);
+      const unsigned w4[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        dzT[(c8 * 8 + 2 * j) * 32 + pp] = (u16)(w4[j] & 0xffffu);
+        dzT[(c8 * 8 + 2 * j + 1) * 32 + pp] = (u16)(w4[j] >> 16);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < COUT)
+      for (int pp = 0; pp < 32; ++pp) bsum += Bf2F(dzT[threadIdx.x * 32 + pp]);
+    bf16x8 a[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const int co = m * 16 + r;
+      a[m] = co < COUT
+                 ? *reinterpret_cast<const bf16x8 *>(dzT + co * 32 + 8 * g)
+                 : Zero8();
+    }
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+      if (wv + 4 * i >= NTN) break;  // wave-uniform
+      bf16x8 b = Zero8();
+      if (lofs[i] >= 0) {
+        const u16 *src = xT + lofs[i] + 8 * g;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) b[j] = __builtin_bit_cast(__bf16, src[j]);
+      }
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+        acc[i][m] =
+            __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b, acc[i][m], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) {
+    const int col = (wv + 4 * i) * 16 + r;
+    if (wv + 4 * i >= NTN || col >= K) continue;
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int co = m * 16 + 4 * g + v;
+        if (co < COUT)
+          part[((long)blockIdx.x * COUT + co) * K + col] = acc[i][m][v];
+      }
+  }
+  if (threadIdx.x < COUT) bpart[(long)blockIdx.x * COUT + threadIdx.x] = bsum;
 }
 
 // Deterministic sum over blocks of part[b][i]: stage 1 sums segment s of
@@ -567,9 +692,15 @@ void WgradT(const u16 *x, const u16 *dz, float *part, float *bpart, int N,
     PS_FATAL("filter gradient: %ld positions exceed 32-bit indexing", P);
   const int chunk = WgradChunk(P);
   const int nb = (int)((P + chunk - 1) / chunk);
-  hipLaunchKernelGGL((WgradKernel<CIN, COUT>), dim3(nb, (NTN + 3) / 4),
-                     dim3(256), 0, Stream(), x, dz, part, bpart, (int)P, D, H,
-                     W, chunk);
+  // rows of a multiple of 32 positions: chunks (multiples of 32) then
+  // split into 32-position steps that never leave a row
+  if (W % 32 == 0)
+    hipLaunchKernelGGL((WgradLdsKernel<CIN, COUT>), dim3(nb), dim3(256), 0,
+                       Stream(), x, dz, part, bpart, (int)P, D, H, W, chunk);
+  else
+    hipLaunchKernelGGL((WgradKernel<CIN, COUT>), dim3(nb, (NTN + 3) / 4),
+                       dim3(256), 0, Stream(), x, dz, part, bpart, (int)P, D,
+                       H, W, chunk);
   Check();
   *nparts = nb;
 }

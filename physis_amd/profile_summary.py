@@ -170,7 +170,8 @@ def unet_md(kt_dir: str, out_md: str, command: str) -> None:
     for f in glob.glob(os.path.join(kt_dir, "*_kernel_stats.csv")):
         for r in csv.DictReader(open(f)):
             m = re.search(r"ConvKernel<(\d+), (\d+), (true|false)>", r["Name"])
-            w = re.search(r"WgradKernel<(\d+), (\d+)>", r["Name"])
+            w = re.search(r"Wgrad(?:Lds)?Kernel<(\d+), (\d+)>",
+                          r["Name"])
             if m:
                 cin, cout = int(m.group(1)), int(m.group(2))
                 what = "forward" if m.group(3) == "true" else "backward data"

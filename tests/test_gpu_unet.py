@@ -74,6 +74,30 @@ def test_conv_backward(ps, dims):
     assert np.max(np.abs(db - rdb)) <= 1e-4 * np.max(np.abs(rdb))
 
 
+# Rows of a multiple of 32 positions take the LDS-staged filter-gradient
+# kernel (the widths above take the gathering one): both channel layouts of
+# every level, a batch of two, and several blocks.
+WGRAD_LDS_DIMS = [(1, 3, 4, 32, 8, 8), (2, 3, 4, 32, 8, 16),
+                  (1, 2, 3, 64, 24, 8), (1, 3, 3, 32, 48, 16),
+                  (1, 2, 3, 32, 16, 48), (1, 2, 2, 32, 32, 32),
+                  (1, 8, 9, 64, 8, 8)]
+
+
+@pytest.mark.parametrize("dims", WGRAD_LDS_DIMS)
+def test_wgrad_lds_rows(ps, dims):
+    from physis_amd.warmstart import conv_backward_filter_host
+    print(f"filter gradient {dims}", flush=True)
+    n, d, h, wd, cin, cout = dims
+    rng = np.random.default_rng(6)
+    x = U.bf16_bits(rng.standard_normal(n * d * h * wd * cin))
+    dy = U.bf16_bits(rng.standard_normal(n * d * h * wd * cout))
+    w = U.bf16_bits(rng.standard_normal(cout * 27 * cin) * 0.2)
+    _, rdw, rdb = U.conv_backward(x, dy, w, *dims)
+    dw, db = conv_backward_filter_host(x, dy, *dims)
+    assert np.max(np.abs(dw - rdw)) <= 1e-4 * np.max(np.abs(rdw))
+    assert np.max(np.abs(db - rdb)) <= 1e-4 * np.max(np.abs(rdb))
+
+
 def test_wgrad_spans_several_blocks(ps):
     """More positions than one filter-gradient block (4096): partials of
     several blocks, the last one ragged."""
