@@ -7,14 +7,15 @@
 //
 // 3x3x3 convolution as an implicit GEMM on v_mfma_f32_16x16x32_bf16:
 //   forward   D[16 x-positions][16 cout] += A[16 pos][32 k] B[32 k][16 cout],
-//             k = tap * CIN + ci, A gathered from the input with zero
-//             padding, B the weights stored [cout][k] (K padded to 32);
+//             k = tap * CIN + ci, A read from an LDS-staged input tile with
+//             zero padding, B the weights stored [cout][k] (K padded to 32);
 //   backward  dX = the forward kernel on dZ with the flipped, transposed
 //             weights Wf[ci][tap' * COUT + co] = W[co][26 - tap'][ci];
 //   filter    dW[co][k] = sum over positions of dZ[p][co] X[p + off(k)][ci]:
-//             GEMM with the positions as K, split over blocks of 4096
-//             positions into FP32 partials summed in block order
-//             (deterministic).
+//             GEMM with the positions as K, split over blocks of up to
+//             16384 positions into FP32 partials summed in block order
+//             (deterministic); both operands staged transposed in LDS for
+//             rows of a multiple of 32 positions.
 // MFMA lane layout (checked on the device against a host product): lane l
 // feeds A row l%16 and B column l%16 with k = 8*(l/16) .. +7, and receives
 // D rows 4*(l/16) .. +3 of column l%16.
@@ -60,24 +61,40 @@ __device__ __forceinline__ bf16x8 Zero8() {
 constexpr int KPad(int cin) { return ((27 * cin + 31) / 32) * 32; }
 
 // ------------------------------------------------------------ convolution
-// One wave: row (n, z, y), in segments of 16 consecutive x, all COUT.  Block:
-// 4 waves on 4 consecutive rows.  Grid: (ceil(H/4), N*D).  When the weight
-// fragments of all K steps fit (KB*NT <= 24 registers of 8 BF16), they are
-// loaded once per wave and reused across the row's segments.
-template <int CIN, int COUT, bool RELU>
+// Block (x tile, 4-row group, n*D + z) copies planes z-1..z+1, rows
+// y0-1..y0+4, x0-1 .. x0+XT of the input (channels-last, zero outside the
+// grid) into LDS with 16-byte loads; wave w then computes row y0+w over the
+// tile in segments of 16 x: D[16 x][16 cout] accumulated over K steps of
+// 32, every A operand (8 channels of one tap at one position) one aligned
+// 16-byte LDS read.  The weight fragments of all K steps stay in registers
+// when there are at most 14 of them (more cost occupancy: 21 for 24 -> 8
+// held 200 VGPRs), else they are read per step (L1-resident).
+template <int CIN, int COUT, bool RELU, int XT>
 __global__ __launch_bounds__(256) void ConvKernel(
     const u16 *__restrict__ x, const u16 *__restrict__ wt,
     const float *__restrict__ bias, u16 *__restrict__ y, int D, int H,
     int W) {
   constexpr int K = 27 * CIN, KB = (K + 31) / 32, KP = KB * 32;
   constexpr int NT = (COUT + 15) / 16;
-  constexpr bool kPre = KB * NT <= 14;  // more costs occupancy (24->8: 2)
+  constexpr bool kPre = KB * NT <= 14;
+  constexpr int XS = XT + 2, NV = CIN / 8;
+  __shared__ __attribute__((aligned(16))) u16 tile[18 * XS * CIN];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int g = lane >> 4, r = lane & 15;
-  const int yy = blockIdx.x * 4 + wv;
+  const int tiles_x = (W + XT - 1) / XT;
+  const int x0 = (blockIdx.x % tiles_x) * XT, y0 = (blockIdx.x / tiles_x) * 4;
   const int nz = blockIdx.y;
   const int n = nz / D, z = nz - n * D;
-  if (yy >= H) return;  // whole wave; the kernel has no barriers
+  for (int v = threadIdx.x; v < 18 * XS * NV; v += 256) {
+    const int c8 = v % NV, t = v / NV;
+    const int xp = t % XS, pr = t / XS;
+    const int zz = z + pr / 6 - 1, yy = y0 + pr % 6 - 1, xx = x0 - 1 + xp;
+    uint4 u = make_uint4(0, 0, 0, 0);
+    if (zz >= 0 && zz < D && yy >= 0 && yy < H && xx >= 0 && xx < W)
+      u = *reinterpret_cast<const uint4 *>(
+          x + ((((long)n * D + zz) * H + yy) * W + xx) * CIN + c8 * 8);
+    *reinterpret_cast<uint4 *>(tile + t * CIN + c8 * 8) = u;
+  }
   bf16x8 bw[kPre ? KB : 1][NT];
   if constexpr (kPre) {
 #pragma unroll
@@ -95,22 +112,25 @@ __global__ __launch_bounds__(256) void ConvKernel(
     const int co = t * 16 + r;
     bb[t] = (bias && co < COUT) ? bias[co] : 0.0f;
   }
+  __syncthreads();
+  const int yy = y0 + wv;
+  if (yy >= H) return;  // whole wave; no barrier follows
   const long row = (((long)n * D + z) * H + yy) * W;
-  for (int x0 = 0; x0 < W; x0 += 16) {
+  for (int xs0 = 0; xs0 < XT && x0 + xs0 < W; xs0 += 16) {
     f32x4 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int xa = x0 + r;
+    const int xa = xs0 + r;
 #pragma unroll 4
     for (int kb = 0; kb < KB; ++kb) {
       const int k0 = kb * 32 + 8 * g;
       bf16x8 a = Zero8();
-      if (k0 < K && xa < W) {
+      if (k0 < K) {
         const int tap = k0 / CIN, ci0 = k0 - tap * CIN;
-        const int zs = z + tap / 9 - 1, ys = yy + (tap / 3) % 3 - 1,
-                  xs = xa + tap % 3 - 1;
-        if (xs >= 0 && xs < W && ys >= 0 && ys < H && zs >= 0 && zs < D)
-          a = Load8(x + ((((long)n * D + zs) * H + ys) * W + xs) * CIN + ci0);
+        // staged plane dz+1 = tap/9, row wv + dy+1, x xa + dx+1
+        a = *reinterpret_cast<const bf16x8 *>(
+            tile + (((tap / 9) * 6 + wv + (tap / 3) % 3) * XS + xa + tap % 3) *
+                       CIN + ci0);
       }
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
@@ -131,7 +151,7 @@ __global__ __launch_bounds__(256) void ConvKernel(
       if (co >= COUT) continue;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int xo = x0 + 4 * g + v;
+        const int xo = x0 + xs0 + 4 * g + v;
         if (xo >= W) continue;
         float val = acc[t][v] + bb[t];
         if (RELU) val = val > 0.0f ? val : 0.0f;
@@ -660,17 +680,34 @@ hipStream_t Stream() { return (hipStream_t)__PSHipStream(); }
 void Check() { PS_HIP_CHECK(hipGetLastError()); }
 unsigned Blocks(long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
+template <int CIN, int COUT, int XT>
+void ConvLaunch(const u16 *x, const u16 *wt, const float *b, bool relu,
+                u16 *y, int N, int D, int H, int W) {
+  const int tiles_x = (W + XT - 1) / XT;
+  const dim3 grid(tiles_x * ((H + 3) / 4), N * D), block(256);
+  if (relu)
+    hipLaunchKernelGGL((ConvKernel<CIN, COUT, true, XT>), grid, block, 0,
+                       Stream(), x, wt, b, y, D, H, W);
+  else
+    hipLaunchKernelGGL((ConvKernel<CIN, COUT, false, XT>), grid, block, 0,
+                       Stream(), x, wt, b, y, D, H, W);
+  Check();
+}
+
+// x tile: 64 positions below 24 input channels, else 32 (measured at
+// batch 8 of 128^3: 8 -> 8 315 us at 64 against 395 at 32; 24 -> 8 1673 us
+// at 32 against 2359 at 64, where the 57 KB tile leaves 2 blocks per CU);
+// 16 for rows of at most 16.
 template <int CIN, int COUT>
 void ConvT(const u16 *x, const u16 *wt, const float *b, bool relu, u16 *y,
            int N, int D, int H, int W) {
-  const dim3 grid((H + 3) / 4, N * D), block(256);
-  if (relu)
-    hipLaunchKernelGGL((ConvKernel<CIN, COUT, true>), grid, block, 0, Stream(),
-                       x, wt, b, y, D, H, W);
+  if constexpr (CIN < 24) {
+    if (W > 32) return ConvLaunch<CIN, COUT, 64>(x, wt, b, relu, y, N, D, H, W);
+  }
+  if (W > 16)
+    ConvLaunch<CIN, COUT, 32>(x, wt, b, relu, y, N, D, H, W);
   else
-    hipLaunchKernelGGL((ConvKernel<CIN, COUT, false>), grid, block, 0,
-                       Stream(), x, wt, b, y, D, H, W);
-  Check();
+    ConvLaunch<CIN, COUT, 16>(x, wt, b, relu, y, N, D, H, W);
 }
 
 // Positions per filter-gradient block: about P / 512 (a multiple of 32),

@@ -21,12 +21,13 @@ from collections import defaultdict
 
 COPY_BYTES = 512 * 2 ** 20
 KERNELS = [  # (label, name fragment, algorithmic bytes per launch)
+    # two-sweep kernels: algorithmic bytes of the two sweeps a launch does
     ("7-pt clamped two-sweep kernel, 512^3 (2 sweeps per launch)",
-     "__PSStencilRun2_diffusion7_kernel<", 8 * 512 ** 3),
+     "__PSStencilRun2_diffusion7_kernel<", 2 * 8 * 512 ** 3),
     ("7-pt clamped run kernel, 512^3", "__PSStencilRun_diffusion7_kernel<",
      8 * 512 ** 3),
     ("7-pt periodic two-sweep kernel, 512^3 (2 sweeps per launch)",
-     "__PSStencilRun2_diffusion7_periodic_kernel<", 8 * 512 ** 3),
+     "__PSStencilRun2_diffusion7_periodic_kernel<", 2 * 8 * 512 ** 3),
     ("19-pt Himeno run kernel, 512x256x256", "__PSStencilRun_jacobi_kernel<",
      56 * 512 * 256 * 256),
     ("PSReduce SUM stage 1, 512^3", "ReduceStage1<float, 2>", 4 * 512 ** 3),

@@ -37,9 +37,13 @@ def _close_bf16(got_bits, ref_bits):
     assert not bad.any(), (int(bad.sum()), np.max(np.abs(g - r)))
 
 
+# (n, d, h, w, cin, cout).  The convolution stages x tiles of 64, 32 or 16
+# positions: widths below, at and above each tile, ragged last tiles and
+# rows not a multiple of 4.
 CONV_DIMS = [(2, 4, 5, 20, 8, 8), (1, 3, 4, 8, 8, 16), (1, 3, 4, 9, 16, 32),
              (1, 3, 5, 7, 48, 16), (1, 4, 4, 16, 24, 8), (1, 2, 3, 33, 32, 32),
-             (1, 3, 3, 6, 16, 16)]
+             (1, 3, 3, 6, 16, 16), (1, 2, 4, 128, 8, 8),
+             (1, 2, 5, 96, 24, 8), (1, 2, 3, 40, 16, 16), (2, 2, 3, 64, 8, 24)]
 
 
 @pytest.mark.parametrize("dims", CONV_DIMS)
