@@ -1,6 +1,13 @@
-"""Summarises rocprofv3 output of physis_amd.profile_kernels.
+"""Summarises rocprofv3 output.
 
     python3 -m physis_amd.profile_summary FETCH_DIR WRITE_DIR KTRACE_DIR OUT.md
+    python3 -m physis_amd.profile_summary --ktrace KTRACE_DIR OUT.md COMMAND
+    python3 -m physis_amd.profile_summary --unet KTRACE_DIR OUT.md COMMAND
+
+The first form summarises physis_amd.profile_kernels (HBM traffic of the
+hot-path kernels); --ktrace tabulates the kernel statistics of any command;
+--unet sets the U-Net convolution kernels of physis_amd.profile_unet
+against their roofline.
 
 FETCH_DIR / WRITE_DIR hold the *_counter_collection.csv of a
 `--pmc FETCH_SIZE` and a `--pmc WRITE_SIZE` run, KTRACE_DIR the
