@@ -103,8 +103,9 @@ def test_wgrad_lds_rows(ps, dims):
 
 
 def test_wgrad_spans_several_blocks(ps):
-    """More positions than one filter-gradient block (4096): partials of
-    several blocks, the last one ragged."""
+    """More positions than one filter-gradient block (256 positions at this
+    size): partials of several blocks, the last one ragged, through the
+    gathering kernel (rows of 40)."""
     from physis_amd.warmstart import conv_backward_filter_host
     dims = (1, 9, 16, 40, 8, 8)  # 5760 positions
     n, d, h, wd, cin, cout = dims
