@@ -283,7 +283,10 @@ __global__ __launch_bounds__(256) void WgradLdsKernel(
   constexpr int K = 27 * CIN, NTN = (K + 15) / 16, MT = (COUT + 15) / 16;
   constexpr int TPW = (NTN + 3) / 4;  // column tiles per wave
   constexpr int XR = 34;              // staged x: x0-1 .. x0+32
-  __shared__ u16 xT[9 * CIN * XR];
+  // xT[row][ci][x] as 16-bit elements, written and read as 32-bit words
+  // (pairs of neighbouring x); 4 words of padding for the last row's
+  // 5-word reads
+  __shared__ unsigned xT[9 * CIN * XR / 2 + 4];
   __shared__ __attribute__((aligned(16))) u16 dzT[COUT * 32];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int g = lane >> 4, r = lane & 15;
@@ -315,20 +318,26 @@ __global__ __launch_bounds__(256) void WgradLdsKernel(
     q /= H;
     const int z = q % D;
     const int n = q / D;
-    for (int v = threadIdx.x; v < 9 * XR * (CIN / 8); v += 256) {
+    // x pairs (xp, xp+1), xp even: one 32-bit word per channel
+    for (int v = threadIdx.x; v < 9 * (XR / 2) * (CIN / 8); v += 256) {
       const int c8 = v % (CIN / 8), t = v / (CIN / 8);
-      const int xp = t % XR, row = t / XR;
+      const int xp = 2 * (t % (XR / 2)), row = t / (XR / 2);
       const int zz = z + row / 3 - 1, yy = y + row % 3 - 1, xx = x0 - 1 + xp;
-      uint4 u = make_uint4(0, 0, 0, 0);
-      if (zz >= 0 && zz < D && yy >= 0 && yy < H && xx >= 0 && xx < W)
-        u = *reinterpret_cast<const uint4 *>(
-            x + ((((long)n * D + zz) * H + yy) * W + xx) * CIN + c8 * 8);
-      const unsigned w4[4] = {u.x, u.y, u.z, u.w};
-      u16 *dst = xT + (row * CIN + c8 * 8) * XR + xp;
+      uint4 u0 = make_uint4(0, 0, 0, 0), u1 = u0;
+      if (zz >= 0 && zz < D && yy >= 0 && yy < H) {
+        const u16 *src =
+            x + ((((long)n * D + zz) * H + yy) * W + xx) * CIN + c8 * 8;
+        if (xx >= 0 && xx < W) u0 = *reinterpret_cast<const uint4 *>(src);
+        if (xx + 1 >= 0 && xx + 1 < W)
+          u1 = *reinterpret_cast<const uint4 *>(src + CIN);
+      }
+      const unsigned a4[4] = {u0.x, u0.y, u0.z, u0.w};
+      const unsigned b4[4] = {u1.x, u1.y, u1.z, u1.w};
+      unsigned *dst = xT + ((row * CIN + c8 * 8) * XR + xp) / 2;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        dst[(2 * j) * XR] = (u16)(w4[j] & 0xffffu);
-        dst[(2 * j + 1) * XR] = (u16)(w4[j] >> 16);
+      for (int j = 0; j < 4; ++j) {  // channels 2j, 2j+1 of the vector
+        dst[(2 * j) * (XR / 2)] = (a4[j] & 0xffffu) | (b4[j] << 16);
+        dst[(2 * j + 1) * (XR / 2)] = (a4[j] >> 16) | (b4[j] & 0xffff0000u);
       }
     }
     for (int v = threadIdx.x; v < 32 * (COUT / 8); v += 256) {
@@ -358,9 +367,23 @@ __global__ __launch_bounds__(256) void WgradLdsKernel(
       if (wv + 4 * i >= NTN) break;  // wave-uniform
       bf16x8 b = Zero8();
       if (lofs[i] >= 0) {
-        const u16 *src = xT + lofs[i] + 8 * g;
+        // elements e .. e+7 from the 5 words that cover them; an odd e
+        // shifts each output word by one element
+        const int e = lofs[i] + 8 * g;
+        const unsigned *wsrc = xT + (e >> 1);
+        unsigned w[5];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) b[j] = __builtin_bit_cast(__bf16, src[j]);
+        for (int k = 0; k < 5; ++k) w[k] = wsrc[k];
+        uint4 o;
+        if (e & 1) {
+          o = make_uint4(__builtin_amdgcn_alignbit(w[1], w[0], 16),
+                         __builtin_amdgcn_alignbit(w[2], w[1], 16),
+                         __builtin_amdgcn_alignbit(w[3], w[2], 16),
+                         __builtin_amdgcn_alignbit(w[4], w[3], 16));
+        } else {
+          o = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+        b = __builtin_bit_cast(bf16x8, o);
       }
 #pragma unroll
       for (int m = 0; m < MT; ++m)
