@@ -710,14 +710,16 @@ Program Parser::ParseProgram() {
       // C++ declarations outside the Physis subset are kept as written:
       // up to the ';' that ends them at brace depth 0, or to the closing
       // brace of a namespace body.  A class name becomes a type name.
-      const bool ns = t.Is("namespace");
+      // A function template ends with its body (no ';' follows).
+      const bool ns = t.Is("namespace"), tmpl = t.Is("template");
       if (t.Is("class") && Peek(1).kind == Tok::kIdent)
         typedefs_.insert(Peek(1).text);
       int depth = 0;
       while (true) {
         if (Peek().kind == Tok::kEnd) Fail("unterminated declaration");
         if (Peek().Is("{")) ++depth;
-        if (Peek().Is("}") && --depth == 0 && ns) {
+        if (Peek().Is("}") && --depth == 0 &&
+            (ns || (tmpl && !Peek(1).Is(";")))) {
           ++pos_;
           break;
         }

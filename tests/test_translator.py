@@ -261,6 +261,8 @@ float first(PSGrid3DFloat g) { return PSGridGet(g, 0, 1, 2); }
 CXX_PROG = """#include <iostream>
 #include "physis/physis.h"
 using namespace std;
+template <typename T> T twice(T v) { return v + v; }
+namespace util { inline int one() { return 1; } }
 class Scale {
  public:
   Scale(int s) { s_ = s; }
@@ -297,6 +299,9 @@ def test_cplusplus_declarations_are_kept():
     assert 'extern "C" {' not in code[:cls]
     assert "void run(Scale & s, PSGrid3DFloat a, PSGrid3DFloat b)" in code
     assert "Scale s(2);" in code and "(float)s()" in code
+    # a function template and a namespace end with their closing brace
+    assert "template <typename T> T twice(T v) { return v + v; }" in code
+    assert code.index("namespace util") < code.index("class Scale")
     with tempfile.TemporaryDirectory() as d:
         src = os.path.join(d, "cxx.hip.cpp")
         open(src, "w").write(code)
