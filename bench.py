@@ -307,9 +307,12 @@ def main() -> int:
         hip.record(e1)
         fused.append(hip.ms(e0, e1) / pairs)
     k_ms = float(np.mean(fused))                  # per two-sweep launch
-    # algorithmic bytes per launch: 8 B/pt per sweep x 2 sweeps
-    alg_launch = 2 * BYTES_PER_PT_7 * npts
-    achieved = alg_launch / (k_ms * 1e-3) / 1e9
+    # A two-sweep launch must read u and write u'' once: its minimum HBM
+    # traffic is one sweep's 8 B/pt, and that is what its roofline counts.
+    # The per-sweep convention (2 sweeps x 8 B/pt per launch, the Gpts/s
+    # view) is reported beside it as effective_GBps.
+    min_launch = BYTES_PER_PT_7 * npts
+    achieved = min_launch / (k_ms * 1e-3) / 1e9
     traffic2 = load_traffic("__PSStencilRun2_diffusion7_kernel")
     dom = (0, N7, 0, N7, 0, N7)
     d7.sweep_domain(g1, g2, dom, DIFFUSION_COEF)
@@ -396,10 +399,13 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic2,
             "kernel_ms": round(k_ms, 5),
-            "algorithmic_bytes_per_launch": int(alg_launch),
-            "note": "one launch performs two sweeps while moving about "
-                    "one sweep's bytes (traffic), so frac can exceed 1; "
-                    "hbm_GBps is the measured traffic over the kernel time",
+            "algorithmic_bytes_per_launch": int(min_launch),
+            "note": "a launch performs two sweeps; its algorithmic bytes are "
+                    "the minimum HBM traffic of the pass (read u, write u'': "
+                    "8 B/pt); effective_GBps counts 8 B/pt per sweep",
+            "effective_GBps": round(2 * min_launch / (k_ms * 1e-3) / 1e9, 1),
+            "traffic_over_minimum": (round(traffic2 / min_launch, 4)
+                                     if traffic2 else None),
             "hbm_GBps": (round(traffic2 / (k_ms * 1e-3) / 1e9, 1)
                          if traffic2 else None),
             "run_ms_per_sweep": round(sweep_ms, 5),
@@ -419,9 +425,12 @@ def main() -> int:
         "periodic_7pt": {
             "value": round(npts * 200 / t_per / 1e9, 2), "unit": "Gpts/s",
             "kernel_ms_per_sweep": round(float(np.mean(pk)), 5),
+            # two sweeps per launch at one sweep's minimum traffic
             "frac": round(BYTES_PER_PT_7 * npts /
-                              (float(np.mean(pk)) * 1e-3) / 1e9 /
-                              HBM_PEAK_GBS, 4)},
+                          (2 * float(np.mean(pk)) * 1e-3) / 1e9 /
+                          HBM_PEAK_GBS, 4),
+            "effective_GBps": round(BYTES_PER_PT_7 * npts /
+                                    (float(np.mean(pk)) * 1e-3) / 1e9, 1)},
         "himeno_19pt": {
             "grid": list(SHAPE19), "sweeps": 400,
             "value": round(n19 * 400 / t19 / 1e9, 2), "unit": "Gpts/s",

@@ -7,8 +7,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <initializer_list>
 
 #include "physis/physis_hip.h"
 #include "physis/hip/stencil.h"
@@ -59,16 +61,51 @@ inline int BlocksPerCU(const void *kernel, int threads) {
   return n;
 }
 
+// Buffer windows (stencil.h: BufWin) address at most 2^31 - 1 bytes, the
+// reach of a 32-bit buffer range.  The largest z chunk for which a window of
+// grid g over the chunk plus `margin` planes on each side stays within that
+// reach; a grid of which not even one such window fits is rejected (the
+// reference's 32-bit PSIndex, include/physis/physis_common.h:33-55, bounds
+// elements, not bytes).
+inline int WindowChunkLimit(const __PSGrid *g, int margin) {
+  if (!g) return INT32_MAX;
+  int64_t plane = (int64_t)g->elm_size;
+  for (int d = 0; d < g->num_dims && d < 2; ++d) plane *= g->dim[d];
+  if (plane <= 0) return INT32_MAX;
+  const int64_t planes = (int64_t)INT32_MAX / plane - 2 * (int64_t)margin;
+  if (planes < 1) {
+    std::fprintf(stderr,
+                 "[physis-hip] a %lld-byte plane (with %d halo planes per "
+                 "side) exceeds the 2 GiB reach of a buffer window\n",
+                 (long long)plane, margin);
+    std::exit(EXIT_FAILURE);
+  }
+  return planes > INT32_MAX ? INT32_MAX : (int)planes;
+}
+inline int WindowChunkLimit(std::initializer_list<const __PSGrid *> grids,
+                            int margin) {
+  int m = INT32_MAX;
+  for (const __PSGrid *g : grids) {
+    const int c = WindowChunkLimit(g, margin);
+    m = c < m ? c : m;
+  }
+  return m;
+}
+
 // Tiles a domain into TX x TY column tiles and picks the number of z chunks.
 // Each block costs (chunk + 2) plane steps (two prologue planes), blocks run
-// in waves of num_cus * per_cu; the chunk count minimising
-// waves * (chunk + 2) is chosen so that the last wave is not left mostly idle.
-inline RunShape PlanRun(const __PSDomain &dom, int tx, int ty, int per_cu) {
+// in waves of `resident` blocks; the chunk count minimising
+// waves * (chunk + 2) is chosen so that the last wave is not left mostly
+// idle.  Chunks never exceed max_chunk planes (the window reach).
+inline RunShape PlanRunFor(const __PSDomain &dom, int tx, int ty,
+                           long resident, int max_chunk = INT32_MAX) {
   RunShape s = {0, 0, 0, 0, 0, 0};
   const int nx = dom.local_max[0] - dom.local_min[0];
   const int ny = dom.local_max[1] - dom.local_min[1];
   const int nz = dom.local_max[2] - dom.local_min[2];
   if (nx <= 0 || ny <= 0 || nz <= 0) return s;
+  if (max_chunk < 1) max_chunk = 1;
+  if (resident < 1) resident = 1;
   // x tiles are anchored at x = 0 so that vector loads stay aligned.
   const int xf = dom.local_min[0] >= 0 ? dom.local_min[0] / tx
                                        : -((-dom.local_min[0] + tx - 1) / tx);
@@ -79,11 +116,11 @@ inline RunShape PlanRun(const __PSDomain &dom, int tx, int ty, int per_cu) {
   s.tiles_x = xl - xf + 1;
   s.tiles_y = (ny + ty - 1) / ty;
   const long xy = (long)s.tiles_x * s.tiles_y;
-  const long resident = (long)__PSHipNumCUs() * per_cu;
   long best_cost = -1;
-  int best_tz = 1;
-  for (int tz = 1; tz <= nz; ++tz) {
+  int best_tz = 0;
+  for (int tz = (nz + max_chunk - 1) / max_chunk; tz <= nz; ++tz) {
     const int chunk = (nz + tz - 1) / tz;
+    if (chunk > max_chunk) continue;
     if ((long)(tz - 1) * chunk >= nz) continue;  // same chunk, fewer blocks
     const long blocks = xy * tz;
     if (blocks > (1L << 30)) break;
@@ -94,10 +131,20 @@ inline RunShape PlanRun(const __PSDomain &dom, int tx, int ty, int per_cu) {
       best_tz = tz;
     }
   }
+  if (best_tz == 0) {
+    std::fprintf(stderr, "[physis-hip] no launch shape fits a domain of "
+                 "%d x %d x %d points\n", nx, ny, nz);
+    std::exit(EXIT_FAILURE);
+  }
   s.tiles_z = best_tz;
   s.zchunk = (nz + best_tz - 1) / best_tz;
   s.blocks = (int)(xy * best_tz);
   return s;
+}
+
+inline RunShape PlanRun(const __PSDomain &dom, int tx, int ty, int per_cu,
+                        int max_chunk = INT32_MAX) {
+  return PlanRunFor(dom, tx, ty, (long)__PSHipNumCUs() * per_cu, max_chunk);
 }
 
 // A 3-D grid argument must be live, of the stencil's element type, and small
@@ -197,14 +244,34 @@ class RunTimer {
   hipEvent_t start_ = nullptr, stop_ = nullptr;
 };
 
+// Scratch slots of the reducing maps of one run function: runtime scratch
+// areas 1..7 (__PSHipScratch), one per map that reduces, in map order, so
+// no two maps of a run share partials.
+class ReduceSlots {
+ public:
+  int Next() {
+    if (next_ > kLast) {
+      std::fprintf(stderr,
+                   "[physis-hip] a PSStencilRun holds more than %d "
+                   "PSStencilMapReduce maps\n", kLast);
+      std::exit(EXIT_FAILURE);
+    }
+    return next_++;
+  }
+
+ private:
+  static constexpr int kLast = 7;
+  int next_ = 1;
+};
+
 // Partial results of a PSStencilMapReduce map inside one run function.  The
 // run kernel of execution i writes one partial per block into Slot(i);
 // After(i) folds every completed batch of executions into the user's array
 // (out[i] = reduction of execution i).  A NULL out makes it a no-op.
 class MapReducer {
  public:
-  MapReducer(void *out, int op, int type, int nblocks, int iter, int map_index,
-             int num_maps)
+  MapReducer(void *out, int op, int type, int nblocks, int iter,
+             ReduceSlots &slots)
       : out_((char *)out), op_(op), type_(type), nblocks_(nblocks),
         iter_(iter) {
     if (!out_ || iter <= 0 || nblocks <= 0) {
@@ -213,8 +280,7 @@ class MapReducer {
     }
     elm_ = (type == PS_DOUBLE || type == PS_LONG) ? 8 : 4;
     batch_ = iter < kBatch ? iter : kBatch;
-    (void)num_maps;
-    base_ = (char *)__PSHipScratch(1 + map_index % 7,
+    base_ = (char *)__PSHipScratch(slots.Next(),
                                    (size_t)batch_ * nblocks_ * elm_);
   }
   void *Slot(int i) const {

@@ -100,8 +100,9 @@ template <typename T, int N> struct CanVec {
 // compiler can wait for exactly the plane it needs (s_waitcnt vmcnt(N))
 // instead of draining all in-flight loads.
 // Offsets passed to the accessors are global linear element offsets; the
-// window start is subtracted here.  Windows are < 2 GiB, so a negative
-// offset and the kDrop sentinel are both out of range.
+// window start is subtracted here.  Windows are < 2 GiB (the host planners
+// cap z chunks so that every window fits: run.h WindowChunkLimit), so a
+// negative offset and the kDrop sentinel are both out of range.
 constexpr int kBufDword3 = 0x00020000;  // raw 32-bit dwords, gfx9 family
 constexpr int kDrop = INT32_MIN;
 // Non-temporal (streaming) cache policy bit of buffer stores on gfx950.

@@ -353,9 +353,16 @@ __global__ void ToStorageKernel(const float *__restrict__ in,
 
 // ------------------------------------------------------------------- host
 // Tiles nx x ny into 64*VX x (BY-2) tiles and picks z chunks per problem
-// minimising waves * (chunk + 2), as the Physis run planner does.
+// minimising waves * (chunk + 2), as the Physis run planner does.  Chunks
+// are capped so that every buffer window (the chunk plus two planes per side
+// of the widest, FP64, field) stays within the 2 GiB reach of a buffer range.
 Plan MakePlan(int nx, int ny, int nzs, int nbatch, int vx, int per_cu) {
   Plan p{};
+  const long max_chunk =
+      (long)INT32_MAX / ((long)nx * ny * (long)sizeof(double)) - 4;
+  if (max_chunk < 1)
+    PS_FATAL("Poisson planes of %d x %d points exceed the reach of a buffer "
+             "window", nx, ny);
   const int tx = kWave * vx, ty = kBY - 2;
   p.tiles_x = (nx + tx - 1) / tx;
   p.tiles_y = (ny + ty - 1) / ty;
@@ -365,6 +372,7 @@ Plan MakePlan(int nx, int ny, int nzs, int nbatch, int vx, int per_cu) {
   long best = -1;
   for (int tz = 1; tz <= nzs; ++tz) {
     const int chunk = (nzs + tz - 1) / tz;
+    if (chunk > max_chunk) continue;
     if ((long)(tz - 1) * chunk >= nzs) continue;
     const long blocks = xy * tz;
     if (blocks > (1L << 28)) break;

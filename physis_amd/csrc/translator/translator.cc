@@ -243,8 +243,56 @@ bool Translator::IsUnusedStencil(const Function &f) const {
   return any(*f.body);
 }
 
+namespace {
+
+bool IsAssignment(const std::string &op) {
+  return op == "=" || op == "+=" || op == "-=" || op == "*=" || op == "/=" ||
+         op == "%=" || op == "&=" || op == "|=" || op == "^=" ||
+         op == "<<=" || op == ">>=";
+}
+
+bool NamesIndex(const KernelInfo &k, const std::string &n) {
+  for (const auto &i : k.index)
+    if (i == n) return true;
+  return false;
+}
+
+bool RebindsIndex(const KernelInfo &k, const Expr &e) {
+  const Expr *l = nullptr;
+  if (e.kind == EK::kBinary && IsAssignment(e.op)) l = e.kids[0].get();
+  if ((e.kind == EK::kUnary || e.kind == EK::kPostfix) &&
+      (e.op == "++" || e.op == "--" || e.op == "&"))
+    l = e.kids[0].get();
+  while (l && l->kind == EK::kParen) l = l->kids[0].get();
+  if (l && l->kind == EK::kIdent && NamesIndex(k, l->text)) return true;
+  for (const auto &c : e.kids)
+    if (c && RebindsIndex(k, *c)) return true;
+  return false;
+}
+
+bool RebindsIndex(const KernelInfo &k, const Stmt &s) {
+  for (const auto &d : s.decls) {
+    if (NamesIndex(k, d.name)) return true;
+    if (d.init && RebindsIndex(k, *d.init)) return true;
+  }
+  for (const Expr *e : {s.expr.get(), s.cond.get(), s.step.get()})
+    if (e && RebindsIndex(k, *e)) return true;
+  if (s.init && RebindsIndex(k, *s.init)) return true;
+  for (const auto &c : s.kids)
+    if (c && RebindsIndex(k, *c)) return true;
+  return false;
+}
+
+}  // namespace
+
+// An index parameter denotes the point's coordinate only while the stencil
+// never redeclares, assigns, increments or takes the address of its name
+// (the reference resolves every reference to its declaration,
+// translator/stencil_analysis.cc:45-49); otherwise gets that use it are
+// irregular.
 bool Translator::RegularIndex(const KernelInfo &k, const Expr &e, int axis,
                               int *off) {
+  if (k.index_rebound) return false;
   const Expr *x = &e;
   while (x->kind == EK::kParen) x = x->kids[0].get();
   const std::string &idx = k.index[axis];
@@ -377,6 +425,7 @@ void Translator::AnalyzeKernel(KernelInfo &k) {
       k.scalars.push_back({f.params[i].type, f.params[i].name});
     }
   }
+  k.index_rebound = RebindsIndex(k, *f.body);
   AnalyzeGets(k, *f.body);
   Decide(k);
   AnalyzeFusion(k);
@@ -594,6 +643,7 @@ void Translator::AnalyzeFusion(KernelInfo &k) {
   auto no = [&](const std::string &w) { k.fuse_why = w; };
   if (!cfg_.two_sweep) return no("HIP_TWO_SWEEP = false");
   if (!k.fast || k.ndims != 3) return no("not a 2.5D run kernel");
+  if (k.index_rebound) return no("an index parameter is rebound");
   if (k.ret_type != "void") return no("a reduction kernel");
   if (k.grids.size() != 2) return no("not exactly two grids");
   int in = -1, out = -1;
@@ -989,7 +1039,7 @@ std::string Translator::EmitStencilStruct(const KernelInfo &k) {
   if (k.used_rb)
     o << "  int __rb;  // -1 all points, 0 red, 1 black, 2 red then black\n";
   o << "};\n\n";
-  std::vector<std::string> params = {"__PSDomain dom"}, init = {"dom"};
+  std::vector<std::string> params = {"__PSDomain __dom"}, init = {"__dom"};
   for (auto &p : k.order) {
     if (p.first == 'g') {
       const std::string &g = k.grids[p.second].name;
@@ -1008,8 +1058,8 @@ std::string Translator::EmitStencilStruct(const KernelInfo &k) {
   }
   if (k.used_rb) init.push_back("-1");
   o << "static struct __PSStencil_" << n << " __PSStencilMap_" << n << "(\n    "
-    << Join(params, ", ") << ") {\n  struct __PSStencil_" << n << " s = {"
-    << Join(init, ", ") << "};\n  return s;\n}\n\n";
+    << Join(params, ", ") << ") {\n  struct __PSStencil_" << n << " __s = {"
+    << Join(init, ", ") << "};\n  return __s;\n}\n\n";
   if (k.used_rb) {
     std::vector<std::string> args;
     for (auto &pp : params) args.push_back(pp.substr(pp.find_last_of(" *") + 1));
@@ -1017,17 +1067,17 @@ std::string Translator::EmitStencilStruct(const KernelInfo &k) {
     for (int c = 0; c < 3; ++c)
       o << "static struct __PSStencil_" << n << " __PSStencilMap" << names[c]
         << "_" << n << "(\n    " << Join(params, ", ") << ") {\n  struct __PSStencil_"
-        << n << " s = __PSStencilMap_" << n << "(" << Join(args, ", ")
-        << ");\n  s.__rb = " << c << ";\n  return s;\n}\n\n";
+        << n << " __s = __PSStencilMap_" << n << "(" << Join(args, ", ")
+        << ");\n  __s.__rb = " << c << ";\n  return __s;\n}\n\n";
   }
   if (k.ret_type != "void") {
     std::vector<std::string> args;
     for (auto &pp : params) args.push_back(pp.substr(pp.find_last_of(" *") + 1));
     o << "static struct __PSStencil_" << n << " __PSStencilMapReduce_" << n
       << "(\n    " << k.ret_type << " *__v, int __op, " << Join(params, ", ")
-      << ") {\n  struct __PSStencil_" << n << " s = __PSStencilMap_" << n << "("
-      << Join(args, ", ") << ");\n  s.__reduce_out = __v;\n"
-      << "  s.__reduce_op = __op;\n  return s;\n}\n\n";
+      << ") {\n  struct __PSStencil_" << n << " __s = __PSStencilMap_" << n << "("
+      << Join(args, ", ") << ");\n  __s.__reduce_out = __v;\n"
+      << "  __s.__reduce_op = __op;\n  return __s;\n}\n\n";
   }
   return o.str();
 }
@@ -1674,7 +1724,8 @@ std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
     << " *mid, hipStream_t stream) {\n"
     << "  static int per_cu = physis_hip::BlocksPerCU(\n"
        "      (const void *)__PSStencilRun2_" << n << "<16>, 64 * 16);\n"
-    << "  const physis_hip::RunShape sh = physis_hip::PlanRun(s.dom, 256, 12, per_cu);\n"
+    << "  const physis_hip::RunShape sh = physis_hip::PlanRun(s.dom, 256, 12, per_cu,\n"
+       "      physis_hip::WindowChunkLimit({s." << I << "}, 3));\n"
     << "  if (sh.blocks == 0) return;\n"
     << "  hipLaunchKernelGGL((__PSStencilRun2_" << n
     << "<16>), dim3(sh.blocks), dim3(64 * 16), 0, stream, " << Join(largs, ", ")
@@ -1762,13 +1813,23 @@ std::string Translator::EmitLauncher(const KernelInfo &k) {
       }
     }
     if (by - 2 * k.hy < 1) Fail(n, "y halo too deep for the run kernel");
+    // buffer windows reach past the z chunk by `margin` planes per side
+    int margin = 0;
+    std::vector<std::string> wgrids;
+    for (auto &g : k.grids) {
+      if (!(g.read || g.emitted) || g.user) continue;
+      wgrids.push_back("s." + g.name);
+      if (g.mode == GridParam::kStaged)
+        margin = std::max(margin, std::max(1 - g.zl, g.zr + 1));
+    }
     o << "template <int VX>\nstatic physis_hip::RunShape __PSShape_" << n
       << "(const struct __PSStencil_" << n << " &s) {\n"
       << "  static int per_cu = physis_hip::BlocksPerCU(\n"
       << "      (const void *)__PSStencilRun_" << n << "<VX, " << by
       << ", -1>, 64 * " << by << ");\n"
       << "  return physis_hip::PlanRun(s.dom, 64 * VX, " << by << " - 2 * "
-      << k.hy << ", per_cu);\n}\n\n";
+      << k.hy << ", per_cu, physis_hip::WindowChunkLimit({" << Join(wgrids, ", ")
+      << "}, " << margin << "));\n}\n\n";
     o << "template <int VX>\nstatic int __PSLaunch_" << n << "(const struct __PSStencil_"
       << n << " &s, hipStream_t stream, void *red) {\n"
       << "  const physis_hip::RunShape sh = __PSShape_" << n << "<VX>(s);\n"
@@ -1871,6 +1932,10 @@ std::string Translator::EmitRunFunction(const RunSite &r) {
     << ") {\n  hipStream_t stream = (hipStream_t)__PSHipStream();\n"
     << "  __PSTraceStencilPre(\"" << Join(r.kernels, ", ") << "\");\n"
     << "  physis_hip::RunTimer timer(stream);\n";
+  bool any_red = false;
+  for (size_t i = 0; i < r.kernels.size(); ++i)
+    any_red = any_red || kernels_.at(r.kernels[i]).ret_type != "void";
+  if (any_red) o << "  physis_hip::ReduceSlots __slots;\n";
   for (size_t i = 0; i < r.kernels.size(); ++i) {
     const KernelInfo &k = kernels_.at(r.kernels[i]);
     std::string si = "s" + std::to_string(i);
@@ -1878,7 +1943,7 @@ std::string Translator::EmitRunFunction(const RunSite &r) {
       o << "  physis_hip::MapReducer red" << i << "(" << si << ".__reduce_out, "
         << si << ".__reduce_op, " << PsTypeOf(k.ret_type) << ",\n      " << si
         << ".__reduce_out ? __PSBlocks_" << k.name << "(" << si
-        << ") : 0, iter, " << i << ", " << r.kernels.size() << ");\n";
+        << ") : 0, iter, __slots);\n";
   }
   const bool fuse = r.kernels.size() == 2 && r.kernels[0] == r.kernels[1] &&
                     kernels_.at(r.kernels[0]).fusable;

@@ -72,6 +72,9 @@ struct KernelInfo {
   std::vector<std::pair<char, int>> order;
   std::string ret_type = "void";
   bool fast = false;          // 2.5D template path
+  // an index parameter is redeclared, assigned, incremented or has its
+  // address taken in the body: every get is treated as irregular
+  bool index_rebound = false;
   std::string slow_reason;    // why not
   int hy = 0;                 // common y halo of staged grids
   bool used_plain = false, used_reduce = false;

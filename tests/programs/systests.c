@@ -563,3 +563,60 @@ float systest_host_get(int x, int y, int z) {
   PSGridFree(g2);
   return v;
 }
+
+/* An index parameter shadowed by a block-scope declaration: the get reads
+ * g at (x + 1, y, z) of the point (x is rebound to x + 2 inside the block).
+ * The translator must not take the inner x for the coordinate. */
+static void kshadow(const int x, const int y, const int z, PSGrid3DFloat g,
+                    PSGrid3DFloat g2) {
+  float v;
+  int xx = x;
+  {
+    int x = xx + 2;
+    v = PSGridGet(g, x - 1, y, z);
+  }
+  PSGridEmit(g2, v);
+}
+
+void systest_shadowed_index(const float *in, float *out, int nx, int ny,
+                            int nz) {
+  PSGrid3DFloat g = PSGrid3DFloatNew(nx, ny, nz);
+  PSGrid3DFloat g2 = PSGrid3DFloatNew(nx, ny, nz);
+  PSDomain3D d = PSDomain3DNew(0, nx, 0, ny, 0, nz);
+  PSGridCopyin(g, in);
+  PSStencilRun(PSStencilMap(kshadow, d, g, g2));
+  PSGridCopyout(g2, out);
+  PSGridFree(g);
+  PSGridFree(g2);
+}
+
+/* Seven reducing maps in one run (the most a run may hold): map i sums
+ * (i + 1) * g over its own domain, so every result differs. */
+static float kscaled(const int x, const int y, const int z, PSGrid3DFloat g,
+                     PSGrid3DFloat g2, float s) {
+  float v = PSGridGet(g, x, y, z) * s;
+  PSGridEmit(g2, v);
+  return v;
+}
+
+void systest_seven_reductions(float *sums) {
+  int n = 16, i;
+  PSGrid3DFloat g = PSGrid3DFloatNew(n, n, n);
+  PSGrid3DFloat g2 = PSGrid3DFloatNew(n, n, n);
+  float *indata = (float *)malloc(sizeof(float) * n * n * n);
+  for (i = 0; i < n * n * n; i++) indata[i] = (float)(i % 7);
+  PSGridCopyin(g, indata);
+  PSDomain3D d0 = PSDomain3DNew(0, n, 0, n, 0, n);
+  PSDomain3D d1 = PSDomain3DNew(0, n, 0, n, 0, n / 2);
+  PSStencilRun(PSStencilMapReduce(&sums[0], PS_SUM, kscaled, d0, g, g2, 1.0f),
+               PSStencilMapReduce(&sums[1], PS_SUM, kscaled, d1, g, g2, 2.0f),
+               PSStencilMapReduce(&sums[2], PS_SUM, kscaled, d0, g, g2, 3.0f),
+               PSStencilMapReduce(&sums[3], PS_SUM, kscaled, d1, g, g2, 4.0f),
+               PSStencilMapReduce(&sums[4], PS_SUM, kscaled, d0, g, g2, 5.0f),
+               PSStencilMapReduce(&sums[5], PS_SUM, kscaled, d1, g, g2, 6.0f),
+               PSStencilMapReduce(&sums[6], PS_SUM, kscaled, d0, g, g2, 7.0f),
+               1);
+  PSGridFree(g);
+  PSGridFree(g2);
+  free(indata);
+}

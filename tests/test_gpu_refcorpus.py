@@ -57,6 +57,15 @@ CORPUS = [
 ]
 
 
+def _require(exe):
+    """The corpus executables are part of the build (__graft_entry__.build()
+    makes them wherever the reference sources are) and travel with the
+    tree; a missing one is a failed build, not a skipped test."""
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} is missing: build with __graft_entry__.build() "
+                    "(make -C oracle ref-hip) before the GPU tests")
+
+
 # Programs physisc-hip does not translate (none at present): it stops with an
 # error naming the feature.
 UNSUPPORTED = set()
@@ -67,9 +76,7 @@ def test_reference_program_on_hip(name):
     if name in UNSUPPORTED:
         pytest.skip(f"{name}: not supported by physisc-hip yet")
     exe = os.path.join(EXE_DIR, name + ".exe")
-    if not os.path.exists(exe):
-        pytest.skip(f"{name}: not built (make -C oracle ref-hip needs the "
-                    "reference sources)")
+    _require(exe)
     print(f"running {name}", flush=True)
     proc = subprocess.run([exe], stdout=subprocess.PIPE,
                           stderr=subprocess.PIPE, timeout=90)
@@ -114,9 +121,7 @@ def _staggered_expected():
 @pytest.mark.parametrize("name", EXAMPLES)
 def test_reference_example_on_hip(name):
     exe = os.path.join(EXE_DIR, "ex_" + name + ".exe")
-    if not os.path.exists(exe):
-        pytest.skip(f"{name}: not built (make -C oracle ref-hip needs the "
-                    "reference sources)")
+    _require(exe)
     print(f"running example {name}", flush=True)
     proc = subprocess.run([exe], stdout=subprocess.PIPE,
                           stderr=subprocess.PIPE, timeout=90)
@@ -136,9 +141,7 @@ def test_reference_example_on_hip(name):
 @pytest.mark.parametrize("size,count", [(32, 4), (24, 10)])
 def test_reference_diffusion_benchmark_on_hip(size, count, tmp_path):
     exe = os.path.join(EXE_DIR, "diffusion3d_physis.exe")
-    if not os.path.exists(exe):
-        pytest.skip("not built (make -C oracle ref-hip needs the reference "
-                    "sources)")
+    _require(exe)
     print(f"running the diffusion benchmark {size}^3 x {count}", flush=True)
     proc = subprocess.run([exe, "--count", str(count), "--size", str(size),
                            "--dump"], stdout=subprocess.PIPE,

@@ -100,3 +100,25 @@ def test_host_set_and_get(prog):
     prog.systest_host_get.argtypes = [C.c_int] * 3
     for x, y, z in ((0, 0, 0), (3, 2, 1), (1, 2, 0)):
         assert prog.systest_host_get(x, y, z) == 2.0 * (x + 4 * y + 12 * z)
+
+
+def test_shadowed_index_reads_the_shifted_point(prog):
+    # kshadow reads g at (x + 1, y, z) through a block-scope x = x + 2; with
+    # the REF linear rule that is element i + 1, and 0 past the last one
+    nx, ny, nz = 40, 6, 5
+    a = np.arange(1, nx * ny * nz + 1, dtype=np.float32)
+    out = np.zeros_like(a)
+    prog.systest_shadowed_index(a.ctypes.data_as(C.c_void_p),
+                                out.ctypes.data_as(C.c_void_p), nx, ny, nz)
+    expect = np.zeros_like(a)
+    expect[:-1] = a[1:]
+    assert np.array_equal(out, expect)
+
+
+def test_seven_reducing_maps_keep_their_own_partials(prog):
+    sums = np.zeros(7, np.float32)
+    prog.systest_seven_reductions(sums.ctypes.data_as(C.c_void_p))
+    g = (np.arange(16 ** 3) % 7).astype(np.float64)
+    full, half = g.sum(), g[:16 * 16 * 8].sum()
+    expect = [(i + 1) * (full if i % 2 == 0 else half) for i in range(7)]
+    assert sums.tolist() == expect

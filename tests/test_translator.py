@@ -311,3 +311,49 @@ def test_cplusplus_declarations_are_kept():
                             "-I" + os.path.join(ROOT, "include"), src],
                            capture_output=True, text=True)
         assert p.returncode == 0, p.stderr[-3000:]
+
+
+@pytest.mark.parametrize("body", [
+    # shadowed in a block
+    "int xx = x; { int x = xx + 2; v = PSGridGet(a, x - 1, y, z); }",
+    # assigned
+    "int xx = x; xx = xx; v = PSGridGet(a, x + 1, y, z); x = x + 0;",
+    # incremented
+    "z++; v = PSGridGet(a, x, y, z - 1);",
+    # a loop variable with an index name
+    "v = 0; for (int y = 0; y < 2; ++y) v += PSGridGet(a, x, y, z);",
+])
+def test_rebound_index_makes_gets_irregular(body):
+    src = """#include "physis/physis.h"
+static void k(int x, int y, int z, PSGrid3DFloat a, PSGrid3DFloat b) {
+  float v;
+  %s
+  PSGridEmit(b, v);
+}
+void run(PSGrid3DFloat a, PSGrid3DFloat b) {
+  PSDomain3D d = PSDomain3DNew(0, 8, 0, 8, 0, 8);
+  PSStencilRun(PSStencilMap(k, d, a, b), PSStencilMap(k, d, b, a), 2);
+}
+""" % body
+    rc, code, err = translate(None, src)
+    assert rc == 0, err
+    assert "a  read from memory, irregular gets" in code
+    assert "ReadRow" not in code
+    assert "two-sweep fusion of map pairs: no (an index parameter is " \
+           "rebound)" in code
+
+
+def test_unrebound_index_stays_regular():
+    src = """#include "physis/physis.h"
+static void k(int x, int y, int z, PSGrid3DFloat a, PSGrid3DFloat b) {
+  int xx = x + 1;
+  PSGridEmit(b, PSGridGet(a, x + 1, y, z) + xx);
+}
+void run(PSGrid3DFloat a, PSGrid3DFloat b) {
+  PSDomain3D d = PSDomain3DNew(0, 8, 0, 8, 0, 8);
+  PSStencilRun(PSStencilMap(k, d, a, b));
+}
+"""
+    rc, code, err = translate(None, src)
+    assert rc == 0, err
+    assert "a  staged x[-1,1]" in code
