@@ -79,37 +79,105 @@ def oracle_lib():
     L.oracle_diffusion7_run_omp.argtypes = ([FP, FP] + [C.c_int] * 5 +
                                             [F] * 7 + [C.c_int])
     L.oracle_diffusion7_run_omp.restype = C.c_int
+    L.oracle_himeno_run.argtypes = [FP, FP, C.c_void_p] + [C.c_int] * 4 + \
+        [F, C.c_void_p]
+    L.oracle_himeno_run_omp.argtypes = [FP, FP, C.c_void_p] + \
+        [C.c_int] * 4 + [F, C.c_int]
+    L.oracle_himeno_run_omp.restype = C.c_int
     return L
 
 
-def cpu_baselines(a: np.ndarray, coef) -> tuple[dict, dict]:
+def host_info() -> dict:
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "affinity_cores": len(os.sched_getaffinity(0)),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
+def omp_threads() -> int:
+    """Threads of the MPI-OpenMP baseline: OMP_NUM_THREADS when the host
+    sets it (the share of host cores given to this GPU), else every core
+    this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baselines(a: np.ndarray, coef) -> dict:
     """The REFERENCE target's sequential loop (1 core) and the MPI-OpenMP
-    target's loop shape (all host cores), both as the oracle's C
-    restatement, on a bounded prefix of the headline run."""
+    target's loop shape (z division units over the host threads, one rank),
+    both as the oracle's C restatement, each on a bounded prefix of its run:
+    7-point clamped and periodic at 512^3, and the 19-point Himeno sweep at
+    512x256x256.  Rates are points per second of the whole sweeps."""
     L = oracle_lib()
+    div = omp_threads()
+    out = {}
     n = a.size
-    sweeps_ref, sweeps_omp = 64, 320
-    g1, g2 = a.copy(), np.empty_like(a)
+
+    def rate(pts, sweeps, secs):
+        return round(pts * sweeps / secs / 1e9, 4)
+
+    for periodic, name, s_ref, s_omp in ((0, "7pt_clamped", 64, 320),
+                                         (1, "7pt_periodic", 8, 48)):
+        g1, g2 = a.copy(), np.empty_like(a)
+        t0 = time.perf_counter()
+        L.oracle_diffusion7_run(g1, g2, N7, N7, N7, s_ref, periodic, *coef)
+        t_ref = time.perf_counter() - t0
+        g1, g2 = a.copy(), np.empty_like(a)
+        t0 = time.perf_counter()
+        th = L.oracle_diffusion7_run_omp(g1, g2, N7, N7, N7, s_omp, periodic,
+                                         *coef, div)
+        t_omp = time.perf_counter() - t0
+        out[name] = {
+            "ref": {"value": rate(n, s_ref, t_ref), "unit": "Gpts/s",
+                    "cores": 1, "kind": "port",
+                    "sample": f"512^3 {name[4:]} 7-pt, {s_ref} sweeps "
+                              f"({t_ref:.1f} s), REFERENCE-target loop order, "
+                              "gcc -O3"},
+            "mpi_openmp": {"value": rate(n, s_omp, t_omp), "unit": "Gpts/s",
+                           "cores": th, "kind": "port",
+                           "sample": f"512^3 {name[4:]} 7-pt, {s_omp} sweeps "
+                                     f"({t_omp:.1f} s), MPI-OpenMP-target z "
+                                     f"division units over {th} threads, "
+                                     "1 rank"}}
+    from physis_amd.configs import himeno_inputs
+    coefs, p = himeno_inputs(*SHAPE19, seed=0)
+    arr = (C.c_void_p * 12)(*[c.ctypes.data for c in coefs])
+    nx, ny, nz = SHAPE19
+    n19 = nx * ny * nz
+    s_ref, s_omp = 24, 120
+    p0, p1 = p.copy(), p.copy()
     t0 = time.perf_counter()
-    L.oracle_diffusion7_run(g1, g2, N7, N7, N7, sweeps_ref, 0, *coef)
+    L.oracle_himeno_run(p0, p1, arr, nx, ny, nz, s_ref, 0.8, None)
     t_ref = time.perf_counter() - t0
-    cores = len(os.sched_getaffinity(0))
-    div = min(cores, 16)
-    g1, g2 = a.copy(), np.empty_like(a)
+    p0, p1 = p.copy(), p.copy()
     t0 = time.perf_counter()
-    threads = L.oracle_diffusion7_run_omp(g1, g2, N7, N7, N7, sweeps_omp, 0,
-                                          *coef, div)
+    th = L.oracle_himeno_run_omp(p0, p1, arr, nx, ny, nz, s_omp, 0.8, div)
     t_omp = time.perf_counter() - t0
-    ref = {"value": round(n * sweeps_ref / t_ref / 1e9, 4), "unit": "Gpts/s",
-           "cores": 1, "kind": "port",
-           "sample": f"512^3 clamped 7-pt, {sweeps_ref} sweeps "
-                     f"({t_ref:.1f} s), REFERENCE-target loop order, gcc -O3"}
-    omp = {"value": round(n * sweeps_omp / t_omp / 1e9, 4), "unit": "Gpts/s",
-           "cores": threads, "kind": "port",
-           "sample": f"512^3 clamped 7-pt, {sweeps_omp} sweeps "
-                     f"({t_omp:.1f} s), MPI-OpenMP-target z division units "
-                     f"over {threads} threads of {cores} host cores, 1 rank"}
-    return ref, omp
+    # the reference's Himeno throughput counts 56 B/pt of its 14 grids
+    out["19pt_himeno"] = {
+        "ref": {"value": rate(n19, s_ref, t_ref), "unit": "Gpts/s",
+                "cores": 1, "kind": "port",
+                "sample": f"512x256x256 Himeno, {s_ref} sweeps "
+                          f"({t_ref:.1f} s), REFERENCE-target loop order, "
+                          "gcc -O3 (the reference's Himeno makefile uses "
+                          "-O2)"},
+        "mpi_openmp": {"value": rate(n19, s_omp, t_omp), "unit": "Gpts/s",
+                       "cores": th, "kind": "port",
+                       "sample": f"512x256x256 Himeno, {s_omp} sweeps "
+                                 f"({t_omp:.1f} s), MPI-OpenMP-target z "
+                                 f"division units over {th} threads, 1 rank"}}
+    out["host"] = host_info()
+    return out
 
 
 def load_traffic(kernel: str):
@@ -443,9 +511,10 @@ def main() -> int:
         line["precision_tier"] = precision_tier(ps, hip)
     if not args.no_cpu:
         log("timing the CPU baselines")
-        ref, omp = cpu_baselines(a, DIFFUSION_COEF)
-        line["cpu_baseline"] = ref
-        line["cpu_baseline_mpi_openmp"] = omp
+        cpu = cpu_baselines(a, DIFFUSION_COEF)
+        line["cpu_baseline"] = cpu["7pt_clamped"]["ref"]
+        line["cpu_baseline_mpi_openmp"] = cpu["7pt_clamped"]["mpi_openmp"]
+        line["cpu_baselines"] = cpu
     print(json.dumps(line), flush=True)
     ps.PSFinalize()
     return 0

@@ -168,6 +168,19 @@ inline void CheckGrid3D(const __PSGrid *g, int type) {
   }
 }
 
+// Device-to-device copy of `bytes` on `stream` (scratch grids of two-sweep
+// runs start as copies of the grid they stand in for); aborts on an error.
+inline void CopyDevice(void *dst, const void *src, size_t bytes,
+                       hipStream_t stream) {
+  const hipError_t e =
+      hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream);
+  if (e != hipSuccess) {
+    std::fprintf(stderr, "[physis-hip] device copy failed: %s\n",
+                 hipGetErrorString(e));
+    std::exit(EXIT_FAILURE);
+  }
+}
+
 // The domain of a map must lie inside every grid the map emits to; the run
 // kernel stores at every domain point.
 // The device view of a runtime grid as a run kernel takes it.  The runtime

@@ -103,6 +103,7 @@ static void st(int storage, void *p, int64_t i, float v) {
 }
 
 void oracle_to_storage(int storage, const float *in, void *out, int64_t n) {
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < n; ++i) st(storage, out, i, in[i]);
 }
 
@@ -119,6 +120,9 @@ void oracle_poisson_sweep(int storage, const void *pin, void *pout,
                           int nbatch, float omega) {
   const int nzs = nz / nbatch;
 #define P(a, b, c) ld(storage, pin, OFF(a, b, c))
+  /* planes are independent (pin is read only): splitting z over threads
+   * leaves every point's arithmetic unchanged */
+#pragma omp parallel for schedule(static)
   for (int z = 0; z < nz; ++z)
     for (int y = 0; y < ny; ++y)
       for (int x = 0; x < nx; ++x) {
@@ -156,6 +160,9 @@ void oracle_poisson_residual(int storage, const void *p, int p64,
   const int nzs = nz / nbatch;
   const int64_t n = (int64_t)nx * ny * nz;
   double *pv = (double *)malloc(sizeof(double) * (size_t)n);
+  /* per-plane partial sums, folded in plane order below */
+  double *ps = (double *)calloc(3 * (size_t)nz, sizeof(double));
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < n; ++i) {
     double v = p64 ? ((const double *)p)[i] : (double)((const float *)p)[i];
     if (p_out) v = v + (double)ld(storage, e, i) * inv_sigma;
@@ -164,6 +171,7 @@ void oracle_poisson_residual(int storage, const void *p, int p64,
   if (p_out) memcpy(p_out, pv, sizeof(double) * (size_t)n);
   for (int q = 0; q < nbatch; ++q) sums[3 * q] = sums[3 * q + 1] = sums[3 * q + 2] = 0.0;
 #define P(a, b, c) pv[OFF(a, b, c)]
+#pragma omp parallel for schedule(static)
   for (int z = 0; z < nz; ++z)
     for (int y = 0; y < ny; ++y)
       for (int x = 0; x < nx; ++x) {
@@ -185,13 +193,21 @@ void oracle_poisson_residual(int storage, const void *p, int p64,
           r = (s0 - P(x, y, z) / c[3]) * c[10];
         }
         if (rhs && sigma != 0.0) st(storage, rhs, o, (float)(r * sigma));
-        double *s = sums + 3 * (z / nzs);
+        double *s = ps + 3 * (size_t)z;
         const double b = c[11] * c[10];
         s[0] += r * r;
         if (fabs(r) > s[1]) s[1] = fabs(r);
         s[2] += b * b;
       }
 #undef P
+  for (int z = 0; z < nz; ++z) {
+    double *s = sums + 3 * (z / nzs);
+    const double *t = ps + 3 * (size_t)z;
+    s[0] += t[0];
+    if (t[1] > s[1]) s[1] = t[1];
+    s[2] += t[2];
+  }
+  free(ps);
   free(pv);
 }
 
@@ -307,6 +323,7 @@ double oracle_poisson_jacobi_fp64(const float *const *cf, double *p, int nx,
       const double *in = (it % 2 == 0) ? p : t;
       double *out = (it % 2 == 0) ? t : p;
 #define P(a, b, c) in[OFF(a, b, c)]
+#pragma omp parallel for schedule(static)
       for (int z = 0; z < nz; ++z)
         for (int y = 0; y < ny; ++y)
           for (int x = 0; x < nx; ++x) {

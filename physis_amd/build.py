@@ -46,6 +46,10 @@ POISSON_SOURCES = [os.path.join(CSRC, "poisson", "mixed_hip.hip")]
 # Learned warm start (U-Net forward/backward/Adam): libphysis_warmstart_hip.
 WARMSTART_SOURCES = [os.path.join(CSRC, "warmstart", "unet_hip.hip")]
 
+# Streaming-bandwidth reference kernels (copy, 13-stream sum) used by the
+# measurement scripts: libphysis_ceiling_hip.
+CEILING_SOURCES = [os.path.join(CSRC, "ceiling", "stream_hip.hip")]
+
 TRANSLATOR_SOURCES = [os.path.join(CSRC, "translator", f) for f in
                       ("lexer.cc", "parser.cc", "translator.cc", "physisc.cc")]
 PHYSISC = os.path.join(PKG, "bin", "physisc-hip")
@@ -110,6 +114,10 @@ def build_warmstart(objdir: str) -> str:
     return _build_extension(WARMSTART_SOURCES, "physis_warmstart_hip", objdir)
 
 
+def build_ceiling(objdir: str) -> str:
+    return _build_extension(CEILING_SOURCES, "physis_ceiling_hip", objdir)
+
+
 def build_translator() -> str:
     os.makedirs(os.path.dirname(PHYSISC), exist_ok=True)
     _run(["g++", "-std=c++17", "-O2", "-Wall", *TRANSLATOR_SOURCES, "-o",
@@ -140,7 +148,8 @@ def build_all(verbose: bool = True) -> list[str]:
     built = [build_translator(), build_runtime(objdir)]
     with ThreadPoolExecutor(max_workers=4) as ex:
         ext = [ex.submit(build_poisson, objdir),
-               ex.submit(build_warmstart, objdir)]
+               ex.submit(build_warmstart, objdir),
+               ex.submit(build_ceiling, objdir)]
         built += list(ex.map(build_program, PROGRAM_SOURCES))
         built += [e.result() for e in ext]
     shutil.rmtree(objdir, ignore_errors=True)

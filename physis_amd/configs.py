@@ -12,6 +12,8 @@ Readings of the configuration text, stated once here:
 """
 from __future__ import annotations
 
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 from .seeded import uniform01
@@ -67,7 +69,7 @@ def gaussian_rhs(nx: int, ny: int, nz: int, count: int, sig_lo: float,
     x = np.arange(nx, dtype=np.float64)
     y = np.arange(ny, dtype=np.float64)
     z = np.arange(nz, dtype=np.float64)
-    out = np.zeros((nz, ny, nx), np.float64)
+    terms = []
     for g in range(count):
         cx, cy, cz = u[5 * g] * nx, u[5 * g + 1] * ny, u[5 * g + 2] * nz
         s = sig_lo + (sig_hi - sig_lo) * u[5 * g + 3]
@@ -76,8 +78,27 @@ def gaussian_rhs(nx: int, ny: int, nz: int, count: int, sig_lo: float,
         gx = np.exp(k * (x - cx) ** 2)
         gy = np.exp(k * (y - cy) ** 2)
         gz = np.exp(k * (z - cz) ** 2)
-        out += a * gz[:, None, None] * (gy[:, None] * gx[None, :])[None]
-    return out.astype(np.float32).reshape(-1)
+        terms.append((a, gz, gy[:, None] * gx[None, :]))
+    out = np.empty((nz, ny, nx), np.float32)
+
+    # Planes are independent: a thread pool sums the Gaussians plane by
+    # plane, in the same order for every plane, so the result does not
+    # depend on the split.
+    def planes(zr):
+        acc = np.zeros((len(zr), ny, nx), np.float64)
+        for a, gz, gxy in terms:
+            acc += a * gz[zr][:, None, None] * gxy[None]
+        out[zr] = acc.astype(np.float32)
+
+    step = max(1, (1 << 21) // max(1, nx * ny))
+    ranges = [np.arange(z0, min(nz, z0 + step)) for z0 in range(0, nz, step)]
+    if len(ranges) == 1:
+        planes(ranges[0])
+    else:
+        from .seeded import _threads
+        with ThreadPoolExecutor(_threads()) as ex:
+            list(ex.map(planes, ranges))
+    return out.reshape(-1)
 
 
 def poisson_problem(nx: int, ny: int, nzs: int, nbatch: int = 1,

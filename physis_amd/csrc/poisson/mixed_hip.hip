@@ -209,6 +209,164 @@ __global__ __launch_bounds__(kWave *kBY) void SweepKernel(
   }
 }
 
+// s0 of the Himeno operator at one point whose neighbours come from p(dx,
+// dy, dz), in the operand order of Rows19::S0.
+template <typename P>
+__device__ __forceinline__ float S0At(const float (&c)[kNumF], P p) {
+  return c[0] * p(0, 0, 1) + c[1] * p(0, 1, 0) + c[2] * p(1, 0, 0) +
+         c[4] * (p(0, 1, 1) - p(0, -1, 1) - p(0, 1, -1) + p(0, -1, -1)) +
+         c[5] * (p(1, 1, 0) - p(1, -1, 0) - p(-1, 1, 0) + p(-1, -1, 0)) +
+         c[6] * (p(1, 0, 1) - p(1, 0, -1) - p(-1, 0, 1) + p(-1, 0, -1)) +
+         c[7] * p(0, 0, -1) + c[8] * p(0, -1, 0) + c[9] * p(-1, 0, 0) + c[11];
+}
+
+// One Jacobi update in storage precision: the value a sweep stores (M) at
+// a point with centre pc, returned as the float the next sweep loads.
+template <typename M>
+__device__ __forceinline__ M UpdateM(const float (&c)[kNumF], float pc,
+                                     float s0, float omega) {
+  const float ss = (s0 * c[3] - pc) * c[10];
+  // bnd = 0 points keep their value whatever their neighbours hold
+  return Convert<M, float>(c[10] != 0.0f ? pc + omega * ss : pc);
+}
+
+// ------------------------------------------------------ two-sweep passes
+// Two Jacobi sweeps per launch (temporal blocking; the technique of the
+// reference's examples/diffusion-benchmark/
+// diffusion3d_cuda_temporal_blocking.cu:78-161): every field is read once
+// per two sweeps.  Ring A holds the source planes s-1..s+1 with a
+// two-point halo, ring B the intermediate planes s-3..s-1 with a one-point
+// halo, as the floats of the storage-type values the first sweep would
+// store (so the pass is bit-identical to two SweepKernel launches).  Step s
+// computes E'(s) on rows y0-1 .. y0+TY (the columns x0-1 and x0+TX by lanes
+// of wave 0) and E''(s-2) on the tile; one barrier per step.  Points with
+// bnd = 0 (every problem's faces, and points outside the grid, whose
+// fields load as 0) keep their value, so the E' values the pass computes
+// beyond the grid feed only points that ignore them.  The fields are read
+// for E' one step ahead and again, two planes later, for E''.
+template <typename M, int VX, int BY>
+__global__ __launch_bounds__(kWave *BY) void TwoSweepKernel(
+    const M *__restrict__ pin, M *__restrict__ pout, Fields<M> f, int nx,
+    int ny, int nz, float omega, Plan pl) {
+  constexpr int TX = kWave * VX, TY = BY - 4;
+  typedef Stage<float, VX, BY, 2, 2, 4, false, M> SA;
+  typedef Stage<float, VX, BY, 1, 1, 4, false> SB;
+  __shared__ __attribute__((aligned(16))) float a_lds[SA::kLdsElems];
+  __shared__ __attribute__((aligned(16))) float b_lds[SB::kLdsElems];
+  const int lane = threadIdx.x % kWave;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const TileCoord tc = TileOf(pl.tiles_x, pl.tiles_y);
+  const ZRange zr = BlockZ(pl, tc.bz);
+  const int zb = zr.zb, ze = zr.ze;
+  if (zb >= ze) return;  // uniform over the block
+  const int x0 = tc.bx * TX, y0 = tc.by * TY;
+  const int yrow = y0 - 2 + wave;
+  const bool mid_row = wave >= 1 && wave < BY - 1;
+  const bool out_row = wave >= 2 && wave < BY - 2 && yrow < ny;
+  const int xs = x0 + lane * VX;
+  const bool hlane = wave == 0 && lane < 2 * (BY - 2);
+  const int hr = 1 + (lane >> 1);
+  const int px = (lane & 1) ? TX : -1;
+  const int hx = x0 + px, hy = y0 - 2 + hr;
+  const int nxy = nx * ny, n = nxy * nz;
+  const int dim[3] = {nx, ny, nz};
+  const BufWin<M> aw =
+      BufWin<M>::Make(pin, n, (long)(zb - 3) * nxy, (long)(ze + 3) * nxy);
+  const BufWin<M> ow =
+      BufWin<M>::Make(pout, n, (long)zb * nxy, (long)ze * nxy);
+  BufWin<M> fw[kNumF];
+#pragma unroll
+  for (int g = 0; g < kNumF; ++g)
+    fw[g] = BufWin<M>::Make(f.g[g], n, (long)(zb - 1) * nxy,
+                            (long)(ze + 1) * nxy);
+  SA st[kPD];
+  const typename SA::Addr ra = SA::Prepare(dim, x0, yrow, lane);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    st[0].Load(aw, pin, dim, ra, zb - 2 + k);
+    st[0].Store(a_lds + k * SA::kPlane, wave, lane);
+  }
+#pragma unroll
+  for (int k = 0; k < kPD; ++k) st[k].Load(aw, pin, dim, ra, zb + 1 + k);
+  M f1[kNumF][2][VX], f2[kNumF][VX];
+  // fields of plane z for E': the lane's VX points on rows y0-1 .. y0+TY,
+  // its halo point in wave 0
+  auto load1 = [&](int z, int slot) {
+    if (wave == 0) {
+      const int o = hlane ? hx + hy * nx + z * nxy : physis_hip::kDrop;
+#pragma unroll
+      for (int g = 0; g < kNumF; ++g)
+        fw[g].template Load<1>(o, *reinterpret_cast<M(*)[1]>(&f1[g][slot][0]));
+    } else {
+      const int o = mid_row ? xs + yrow * nx + z * nxy : physis_hip::kDrop;
+#pragma unroll
+      for (int g = 0; g < kNumF; ++g)
+        fw[g].template Load<VX>(o, f1[g][slot]);
+    }
+  };
+  load1(zb - 1, 0);
+  __syncthreads();
+  for (int sc = zb - 1; sc <= ze + 1; sc += 4) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int s = sc + k;
+      const int cur = k & 1, nxt = cur ^ 1;
+      if (s > ze + 1) break;  // uniform over the block
+      load1(s + 1, nxt);
+      const int o2 = out_row && s - 2 >= zb ? xs + yrow * nx + (s - 2) * nxy
+                                            : physis_hip::kDrop;
+#pragma unroll
+      for (int g = 0; g < kNumF; ++g)
+        fw[g].template Load<VX, physis_hip::kStreamLoad>(o2, f2[g]);
+      if (mid_row && s <= ze) {
+        Rows19<float, VX> r;
+        r.template Read<SA>(a_lds, k, wave, lane);
+        float *brow = b_lds + RingSlot<4>(k, 3) * SB::kPlane + wave * SB::kRow +
+                      SB::kPad + lane * VX;
+#pragma unroll
+        for (int i = 0; i < VX; ++i) {
+          float c[kNumF];
+#pragma unroll
+          for (int g = 0; g < kNumF; ++g)
+            c[g] = Convert<float, M>(f1[g][cur][i]);
+          brow[i] = Convert<float, M>(
+              UpdateM<M>(c, r.Centre(i), r.S0(c, i), omega));
+        }
+      }
+      if (hlane && s <= ze) {
+        const auto p = [&](int dx, int dy, int dz) {
+          return a_lds[RingSlot<4>(k, dz + 1) * SA::kPlane +
+                       (hr + dy) * SA::kRow + SA::kPad + px + dx];
+        };
+        float c[kNumF];
+#pragma unroll
+        for (int g = 0; g < kNumF; ++g) c[g] = Convert<float, M>(f1[g][cur][0]);
+        b_lds[RingSlot<4>(k, 3) * SB::kPlane + hr * SB::kRow + SB::kPad + px] =
+            Convert<float, M>(UpdateM<M>(c, p(0, 0, 0), S0At(c, p), omega));
+      }
+      if (out_row && s - 2 >= zb) {
+        Rows19<float, VX> r;
+        r.template Read<SB>(b_lds, k, wave, lane);
+        M out[VX];
+        unsigned mask = 0;
+#pragma unroll
+        for (int i = 0; i < VX; ++i) {
+          float c[kNumF];
+#pragma unroll
+          for (int g = 0; g < kNumF; ++g) c[g] = Convert<float, M>(f2[g][i]);
+          out[i] = UpdateM<M>(c, r.Centre(i), r.S0(c, i), omega);
+          if (xs + i < nx) mask |= 1u << i;
+        }
+        physis_hip::StoreMasked<M, VX>(ow, xs + yrow * nx + (s - 2) * nxy,
+                                       out, mask);
+      }
+      st[k % kPD].Store(a_lds + RingSlot<4>(k, 3) * SA::kPlane, wave, lane);
+      __syncthreads();
+      st[k % kPD].Load(aw, pin, dim, ra, s + 2 + kPD);
+    }
+  }
+}
+
 // ----------------------------------------------------------- FP64 residual
 // PT: element type of p (float or double).  ME: storage type of E and RHS.
 // UPDATE: p' = p + inv_sigma * E formed on load, written to pout, residual
@@ -356,14 +514,15 @@ __global__ void ToStorageKernel(const float *__restrict__ in,
 // minimising waves * (chunk + 2), as the Physis run planner does.  Chunks
 // are capped so that every buffer window (the chunk plus two planes per side
 // of the widest, FP64, field) stays within the 2 GiB reach of a buffer range.
-Plan MakePlan(int nx, int ny, int nzs, int nbatch, int vx, int per_cu) {
+Plan MakePlan(int nx, int ny, int nzs, int nbatch, int vx, int per_cu,
+              int ty = kBY - 2) {
   Plan p{};
   const long max_chunk =
       (long)INT32_MAX / ((long)nx * ny * (long)sizeof(double)) - 4;
   if (max_chunk < 1)
     PS_FATAL("Poisson planes of %d x %d points exceed the reach of a buffer "
              "window", nx, ny);
-  const int tx = kWave * vx, ty = kBY - 2;
+  const int tx = kWave * vx;
   p.tiles_x = (nx + tx - 1) / tx;
   p.tiles_y = (ny + ty - 1) / ty;
   p.nzs = nzs;
@@ -404,6 +563,8 @@ struct __PSPoisson {
   double *host_partials = nullptr;
   int sweep_vx;
   Plan sweep_plan, res_plan;
+  int sweep2_vx;     // two-sweep passes
+  Plan sweep2_plan;
 };
 
 namespace {
@@ -454,21 +615,60 @@ void LaunchSweep(const __PSPoisson *s, const M *in, M *out, Fields<M> f,
   CheckLaunch();
 }
 
+// Waves per block of the two-sweep kernel (TY = BY - 4 output rows) and its
+// points per lane.  The fields take 2 * VX values of E' in flight and VX of
+// E'' per lane: at VX = 4 an 8-wave block (two waves per SIMD, up to 256
+// VGPRs) holds them without spills for every storage type.
+template <typename M> struct TwoSweepBY { static constexpr int value = 8; };
+inline int TwoSweepVX(int nx) { return (nx % 4 == 0 && nx > 128) ? 4 : 2; }
+
 template <typename M>
-void InnerSweeps(__PSPoisson *s, int n, float omega) {
-  const Fields<M> f = InnerFields<M>(s);
-  for (int i = 0; i < n; i += 2) {
-    LaunchSweep<M>(s, (const M *)s->e[0], (M *)s->e[1], f, omega);
-    LaunchSweep<M>(s, (const M *)s->e[1], (M *)s->e[0], f, omega);
+const void *TwoSweepFn(int vx) {
+  constexpr int BY = TwoSweepBY<M>::value;
+  return vx == 4 ? (const void *)TwoSweepKernel<M, 4, BY>
+                 : (const void *)TwoSweepKernel<M, 2, BY>;
+}
+
+// Two sweeps in -> out in one launch.
+template <typename M>
+void LaunchTwoSweep(const __PSPoisson *s, const M *in, M *out, Fields<M> f,
+                    float omega) {
+  constexpr int BY = TwoSweepBY<M>::value;
+  const dim3 grid(s->sweep2_plan.blocks), block(kWave * BY);
+  if (s->sweep2_vx == 4)
+    hipLaunchKernelGGL((TwoSweepKernel<M, 4, BY>), grid, block, 0, Stream(),
+                       in, out, f, s->nx, s->ny, s->nz, omega, s->sweep2_plan);
+  else
+    hipLaunchKernelGGL((TwoSweepKernel<M, 2, BY>), grid, block, 0, Stream(),
+                       in, out, f, s->nx, s->ny, s->nz, omega, s->sweep2_plan);
+  CheckLaunch();
+}
+
+// n (even) sweeps a -> t -> a ... ending in a: pairs of sweeps run as
+// two-sweep passes a -> t, t -> a; an odd pair count starts with one pair
+// of single sweeps.
+template <typename M>
+void SweepPairs(__PSPoisson *s, M *a, M *t, Fields<M> f, int n,
+                float omega) {
+  int pairs = n / 2;
+  if (pairs % 2) {
+    LaunchSweep<M>(s, a, t, f, omega);
+    LaunchSweep<M>(s, t, a, f, omega);
+    --pairs;
+  }
+  for (int i = 0; i < pairs; i += 2) {
+    LaunchTwoSweep<M>(s, a, t, f, omega);
+    LaunchTwoSweep<M>(s, t, a, f, omega);
   }
 }
 
+template <typename M>
+void InnerSweeps(__PSPoisson *s, int n, float omega) {
+  SweepPairs<M>(s, (M *)s->e[0], (M *)s->e[1], InnerFields<M>(s), n, omega);
+}
+
 void PlainSweeps(__PSPoisson *s, float *p, int n, float omega) {
-  const Fields<float> f = MasterFields(s);
-  for (int i = 0; i < n; i += 2) {
-    LaunchSweep<float>(s, p, s->t32, f, omega);
-    LaunchSweep<float>(s, s->t32, p, f, omega);
-  }
+  SweepPairs<float>(s, p, s->t32, MasterFields(s), n, omega);
 }
 
 template <typename PT, typename ME, bool UPDATE, bool WRHS>
@@ -672,6 +872,16 @@ __PSPoisson *PSPoissonNew(__PSGrid *const coef[12], int nbatch, int storage) {
   per_cu_res = physis_hip::BlocksPerCU(
       (const void *)ResidualKernel<double, bf16_t, 2, true, true>, kWave * kBY);
   s->sweep_plan = MakePlan(nx, ny, s->nzs, nbatch, s->sweep_vx, per_cu_sweep);
+  {
+    s->sweep2_vx = TwoSweepVX(nx);
+    const void *fn = storage == PS_STORAGE_BF16   ? TwoSweepFn<bf16_t>(s->sweep2_vx)
+                     : storage == PS_STORAGE_FP16 ? TwoSweepFn<fp16_t>(s->sweep2_vx)
+                                                  : TwoSweepFn<float>(s->sweep2_vx);
+    const int by = storage == PS_STORAGE_FP32 ? TwoSweepBY<float>::value
+                                              : TwoSweepBY<bf16_t>::value;
+    s->sweep2_plan = MakePlan(nx, ny, s->nzs, nbatch, s->sweep2_vx,
+                              physis_hip::BlocksPerCU(fn, kWave * by), by - 4);
+  }
   s->res_plan = MakePlan(nx, ny, s->nzs, nbatch, 2, per_cu_res);
   s->partials = (double *)DevAlloc(sizeof(double) * 3 * s->res_plan.blocks);
   ok = ok && s->partials;

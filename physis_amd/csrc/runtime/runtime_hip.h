@@ -38,7 +38,7 @@ struct Runtime {
   int device = 0;
   int num_cus = 0;
   hipStream_t stream = nullptr;  // null stream: the reference's default
-  static constexpr int kScratchSlots = 9;
+  static constexpr int kScratchSlots = 10;
   void *scratch[kScratchSlots] = {};  // partial results, grown on demand
   size_t scratch_bytes[kScratchSlots] = {};
   void *host_result = nullptr;   // pinned landing slot for scalar results

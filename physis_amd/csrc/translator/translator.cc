@@ -638,24 +638,62 @@ struct FuseScan {
 
 }  // namespace
 
+namespace {
+int CountEmits(const Expr &e) {
+  int n = 0;
+  if (const std::string *c = CalleeName(e))
+    if (*c == "PSGridEmit") ++n;
+  for (const auto &kid : e.kids)
+    if (kid) n += CountEmits(*kid);
+  return n;
+}
+int CountEmits(const Stmt &s) {
+  int n = 0;
+  for (const auto &d : s.decls)
+    if (d.init) n += CountEmits(*d.init);
+  for (const Expr *e : {s.expr.get(), s.cond.get(), s.step.get()})
+    if (e) n += CountEmits(*e);
+  if (s.init) n += CountEmits(*s.init);
+  for (const auto &c : s.kids)
+    if (c) n += CountEmits(*c);
+  return n;
+}
+}  // namespace
+
+// Two modes.  Whole grid: the pair covers the whole grid and the u' values
+// the pass computes at halo points outside the grid are provably unused
+// (FuseScan).  Domain: the pair covers a domain strictly inside the grid;
+// the pass computes u' = k(u) at domain points and takes the emitted
+// grid's own values elsewhere (a map leaves points outside its domain
+// unchanged), so k needs no boundary analysis and may read further
+// centre-only grids and return a value to reduce.
 void Translator::AnalyzeFusion(KernelInfo &k) {
   k.fusable = false;
+  k.fuse_domain = false;
   auto no = [&](const std::string &w) { k.fuse_why = w; };
   if (!cfg_.two_sweep) return no("HIP_TWO_SWEEP = false");
   if (!k.fast || k.ndims != 3) return no("not a 2.5D run kernel");
   if (k.index_rebound) return no("an index parameter is rebound");
-  if (k.ret_type != "void") return no("a reduction kernel");
-  if (k.grids.size() != 2) return no("not exactly two grids");
-  int in = -1, out = -1;
-  for (int i = 0; i < 2; ++i) {
+  int in = -1, out = -1, direct = 0;
+  for (int i = 0; i < (int)k.grids.size(); ++i) {
     const GridParam &g = k.grids[i];
-    if (g.read && !g.emitted) in = i;
-    if (g.emitted && !g.read) out = i;
+    if (g.user) return no("a grid of user-defined points");
+    if (g.read && g.emitted) return no("a grid both read and emitted");
+    if (g.emitted) {
+      if (out >= 0) return no("more than one emitted grid");
+      out = i;
+    } else if (g.read && g.mode == GridParam::kStaged) {
+      if (in >= 0) return no("more than one grid read at offsets");
+      in = i;
+    } else if (g.read) {
+      if (g.mode != GridParam::kDirect || g.irregular)
+        return no("grid " + g.name + " is read irregularly");
+      ++direct;
+    }
   }
   if (in < 0 || out < 0) return no("not one read grid and one emitted grid");
   const GridParam &g = k.grids[in];
-  if (g.mode != GridParam::kStaged || g.irregular)
-    return no("the read grid is not staged");
+  if (g.irregular) return no("the read grid is not staged");
   if (g.elem != "float" || k.grids[out].elem != "float")
     return no("not float grids");
   if (g.hx > 1 || g.hy > 1 || g.zl < -1 || g.zr > 1)
@@ -665,12 +703,7 @@ void Translator::AnalyzeFusion(KernelInfo &k) {
     if ((o[0] || o[1] || o[2]) && g.offset_periodic[i] != g.periodic)
       return no("periodic and non-periodic neighbour gets mixed");
   }
-  FuseScan scan(k);
-  scan.Assigns(*k.fn->body);
-  std::vector<std::pair<int, int>> facts;
-  scan.Gets(*k.fn->body, facts, g.periodic);
-  if (!scan.ok) return no(scan.why);
-  if (scan.emits != 1) return no("not exactly one emit");
+  if (CountEmits(*k.fn->body) != 1) return no("not exactly one emit");
   bool top = false;
   for (const auto &st : k.fn->body->kids)
     if (st && st->kind == SK::kExpr && st->expr) {
@@ -678,9 +711,24 @@ void Translator::AnalyzeFusion(KernelInfo &k) {
       top = top || (c && *c == "PSGridEmit");
     }
   if (!top) return no("the emit is conditional");
-  k.fusable = true;
   k.fuse_in = in;
   k.fuse_out = out;
+  std::string whole_why;
+  if (direct == 0 && k.ret_type == "void") {
+    FuseScan scan(k);
+    scan.Assigns(*k.fn->body);
+    std::vector<std::pair<int, int>> facts;
+    scan.Gets(*k.fn->body, facts, g.periodic);
+    if (scan.ok) {
+      k.fusable = true;
+      return;
+    }
+    whole_why = scan.why;
+  }
+  if (g.periodic)
+    return no(whole_why.empty() ? "periodic gets over a domain" : whole_why);
+  k.fusable = true;
+  k.fuse_domain = true;
 }
 
 void Translator::Decide(KernelInfo &k) {
@@ -1114,7 +1162,10 @@ std::string Translator::EmitContext(const KernelInfo &k) {
   if (k.used_rb) o << "//   red-black: " << k.rb_note << "\n";
   if (k.fast)
     o << "//   two-sweep fusion of map pairs: "
-      << (k.fusable ? std::string("yes") : "no (" + k.fuse_why + ")") << "\n";
+      << (k.fusable ? std::string(k.fuse_domain
+                                        ? "yes, over a domain inside the grid"
+                                        : "yes")
+                    : "no (" + k.fuse_why + ")") << "\n";
   o << "struct __PSCtx_" << k.name << " {\n";
   for (auto &g : k.grids) {
     if (g.user) {
@@ -1918,8 +1969,399 @@ std::string Translator::EmitKernel(KernelInfo &k) {
   s += EmitDeviceFunction(k);
   s += k.fast ? EmitFastRunKernel(k) : EmitGenericRunKernel(k);
   s += EmitLauncher(k);
-  if (k.fusable) s += EmitFusedRunKernel(k);
+  if (k.fusable)
+    s += k.fuse_domain ? EmitFusedDomainKernel(k) : EmitFusedRunKernel(k);
   return s;
+}
+
+// The run-function branch of a fusable pair (s0, s1) = k(a -> b), k(b -> a):
+// every pair of the run is one two-sweep pass when __fuse holds.
+std::string Translator::EmitFusedRun(const KernelInfo &k) {
+  std::ostringstream o;
+  const GridParam &gi = k.grids[k.fuse_in], &go = k.grids[k.fuse_out];
+  const std::string V = gi.dev_view, T = gi.elem;
+  o << "  int i0 = 0;\n  if (__fuse) {\n"
+    << "    physis_hip::CheckGrid3D(s0." << gi.name << ", " << gi.ps_type << ");\n"
+    << "    physis_hip::CheckGrid3D(s0." << go.name << ", " << go.ps_type << ");\n"
+    << "    " << V << " va = physis_hip::DevView<" << V << ">(s0." << gi.name << ");\n"
+    << "    " << V << " vb = physis_hip::DevView<" << V << ">(s0." << go.name << ");\n"
+    << "    const size_t bytes = (size_t)s0." << gi.name << "->num_elms * sizeof("
+    << T << ");\n"
+    << "    " << V << " vt = va;\n"
+    << "    vt.p = (" << T << " *)__PSHipScratch(8, bytes);\n";
+  if (!k.fuse_domain) {
+    // Each pass is one pair.  Even counts ping-pong a -> t -> a ...; odd
+    // counts (>= 3) send the first pass to b instead, a -> b -> t -> a ...
+    // (b is free until the last pass writes it).  Either way the last pass
+    // reads t and writes a, and its intermediate sweep to b.
+    o << "    " << V << " src = va;\n"
+      << "    int j = 0;\n"
+      << "    if (iter % 2) {\n"
+      << "      __PSLaunch2_" << k.name << "(s0, va, vb, NULL, stream);\n"
+      << "      src = vb;\n      j = 1;\n    }\n"
+      << "    for (; j < iter; ++j) {\n"
+      << "      const " << V << " dst = ((iter - 1 - j) % 2 == 0) ? va : vt;\n"
+      << "      __PSLaunch2_" << k.name
+      << "(s0, src, dst, j == iter - 1 ? vb.p : NULL, stream);\n"
+      << "      src = dst;\n    }\n";
+  } else {
+    // Points outside the domain keep a's values in a, so the scratch grids
+    // the passes write start as copies of a (stores stay inside the
+    // domain).  Even counts ping-pong a -> t -> a ...; odd counts (>= 3) go
+    // a -> t -> t2 -> a -> t -> a ... (b's values outside the domain are
+    // read by every pass, so b cannot hold a pass).  The last pass writes a,
+    // and its intermediate sweep to b.
+    const bool red = k.ret_type != "void";
+    o << "    " << V << " vt2 = va;\n"
+      << "    physis_hip::CopyDevice(vt.p, va.p, bytes, stream);\n"
+      << "    if (iter % 2) {\n"
+      << "      vt2.p = (" << T << " *)__PSHipScratch(9, bytes);\n"
+      << "      physis_hip::CopyDevice(vt2.p, va.p, bytes, stream);\n    }\n"
+      << "    " << V << " src = va;\n"
+      << "    for (int j = 0; j < iter; ++j) {\n"
+      << "      " << V << " dst = (j % 2 == (iter % 2 ? 0 : 1)) ? va : vt;\n"
+      << "      if (iter % 2 && j < 2) dst = j == 0 ? vt : vt2;\n"
+      << "      __PSLaunch2_" << k.name << "(s0, src, dst, j == iter - 1, stream, "
+      << (red ? "red0.Slot(j), red1.Slot(j)" : "NULL, NULL") << ");\n";
+    if (red) o << "      red0.After(j);\n      red1.After(j);\n";
+    o << "      src = dst;\n    }\n";
+  }
+  o << "    i0 = iter;\n  }\n";
+  return o.str();
+}
+
+// Two-sweep run kernel of a map pair over a domain strictly inside the grid
+// (the Himeno pair, reference examples/himeno/himenobmtxpa_physis.c:
+// 379-385; the technique of examples/diffusion-benchmark/
+// diffusion3d_cuda_temporal_blocking.cu:78-161).  Layout as the whole-grid
+// kernel: ring A holds source planes s-1..s+1 with a two-point x halo, ring
+// B the intermediate planes with a one-point halo, u'(s) is computed on
+// rows y0-1 .. y0+TY (the halo columns by lanes of wave 0), u''(s-2) on the
+// tile.  u' at points outside the domain is the emitted grid's own value
+// (read from it), u' inside the domain is k(src); centre-only grids are
+// read for u' one step ahead and again for u'' at the top of its step.
+std::string Translator::EmitFusedDomainKernel(const KernelInfo &k) {
+  std::ostringstream o;
+  const std::string &n = k.name;
+  const GridParam &gi = k.grids[k.fuse_in], &go = k.grids[k.fuse_out];
+  const std::string T = gi.elem, V = gi.dev_view, I = gi.name, O = go.name;
+  const bool red = k.ret_type != "void";
+  const std::string RT = red ? k.ret_type : "float";
+  std::vector<const GridParam *> direct;
+  for (auto &g : k.grids)
+    if (g.mode == GridParam::kDirect && g.read) direct.push_back(&g);
+  std::vector<std::string> sp, sa;
+  for (auto &sc : k.scalars) {
+    sp.push_back(sc.type + " " + sc.name);
+    sa.push_back(sc.name);
+  }
+  auto rows = RowsOf(gi);
+  auto fill = [&](const std::string &pre, const std::string &ind) {
+    std::ostringstream f;
+    for (size_t i = 0; i < gi.offsets.size(); ++i) {
+      const auto &off = gi.offsets[i];
+      for (size_t r = 0; r < rows.size(); ++r)
+        if (rows[r].dy == off[1] && rows[r].dz == off[2])
+          f << ind << "ctx." << I << "[" << i << "] = " << pre << "_r" << r
+            << "[i + " << off[0] - rows[r].xl << "];\n";
+    }
+    return f.str();
+  };
+  auto read_rows = [&](const std::string &pre, const std::string &lds,
+                       const std::string &S, const std::string &base,
+                       const std::string &ind) {
+    std::ostringstream f;
+    for (size_t r = 0; r < rows.size(); ++r) {
+      const Row &w = rows[r];
+      f << ind << T << " " << pre << "_r" << r << "[VX + " << w.xr - w.xl
+        << "];\n";
+      f << ind << S << "::template ReadRow<" << w.xl << ", " << w.xr << ">("
+        << lds << " + physis_hip::RingSlot<4>(" << base << ", " << w.dz + 1
+        << ") * " << S << "::kPlane, wave + " << w.dy << ", lane, " << pre
+        << "_r" << r << ");\n";
+    }
+    return f.str();
+  };
+  std::vector<std::string> args = {"__PSDomain dom", V + " src", V + " dst",
+                                   V + " bg", "int store_mid"};
+  for (auto *g : direct) args.push_back(g->dev_view + " " + g->name);
+  for (auto &x : sp) args.push_back(x);
+  args.push_back("physis_hip::RunShape sh");
+  args.push_back(RT + " *__restrict__ red1");
+  args.push_back(RT + " *__restrict__ red2");
+  std::vector<std::string> kargs_scalars;
+  for (auto &a : sa) kargs_scalars.push_back(a);
+  const std::string fa = kargs_scalars.empty() ? "" : ", " + Join(kargs_scalars, ", ");
+  const std::string ACC = "physis_hip::ReduceOp<" + RT + ", RED < 0 ? 0 : RED>";
+  o << "// Two-sweep run kernel of " << n << " over a domain strictly inside\n"
+       "// the grid: one pass over src computes u' = " << n << "(src) on the\n"
+       "// tile and a one-point halo (bg's own values outside the domain) and\n"
+       "// u'' = " << n << "(u') two planes behind; dst <- u'', and bg <- u'\n"
+       "// when store_mid.  red1 / red2: per-block partials of the two sweeps.\n";
+  o << "template <int VX, int BY, int RED>\n__global__ __launch_bounds__(64 * BY) "
+       "void __PSStencilRun2_" << n << "(\n    " << Join(args, ", ") << ") {\n";
+  o << "  constexpr int PD = 2, TX = 64 * VX, TY = BY - 4;\n"
+    << "  typedef physis_hip::Stage<" << T << ", VX, BY, 2, 2, 4, false> SA;\n"
+    << "  typedef physis_hip::Stage<" << T << ", VX, BY, 1, 1, 4, false> SB;\n"
+    << "  __shared__ __attribute__((aligned(16))) " << T << " a_lds[SA::kLdsElems];\n"
+    << "  __shared__ __attribute__((aligned(16))) " << T << " b_lds[SB::kLdsElems];\n"
+    << "  const int lane = threadIdx.x % 64;\n"
+       "  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);\n"
+       "  const physis_hip::TileCoord tc = physis_hip::TileOf(sh.tiles_x, sh.tiles_y);\n"
+       "  const int x0 = (sh.x_first_tile + tc.bx) * TX;\n"
+       "  const int y0 = dom.local_min[1] + tc.by * TY;\n"
+       "  const int zb = dom.local_min[2] + tc.bz * sh.zchunk;\n"
+       "  const int ze = min(zb + sh.zchunk, dom.local_max[2]);\n"
+    << "  " << RT << " acc1 = 0, acc2 = 0;\n"
+    << "  if constexpr (RED >= 0) acc1 = acc2 = " << ACC << "::identity();\n"
+    << "  if (zb < ze) {\n"
+       "    const int yrow = y0 - 2 + wave;\n"
+       "    const bool mid_row = wave >= 1 && wave < BY - 1;\n"
+       "    const bool out_row = wave >= 2 && wave < BY - 2 && yrow < dom.local_max[1];\n"
+       "    const int xs = x0 + lane * VX;\n"
+       "    const int nx = src.dim[0], nxy = src.dim[0] * src.dim[1];\n"
+       "    const int n = nxy * src.dim[2];\n"
+       "    if constexpr (VX > 1) {\n"
+       "      __builtin_assume((xs & (VX - 1)) == 0);\n"
+       "      __builtin_assume((nx & 3) == 0);\n    }\n"
+       "    // u' halo columns x0 - 1 and x0 + TX of rows y0 - 1 .. y0 + TY: one\n"
+       "    // point per lane of wave 0 (which otherwise only loads)\n"
+       "    const bool hlane = wave == 0 && lane < 2 * (BY - 2);\n"
+       "    const int hr = 1 + (lane >> 1);\n"
+       "    const int px = (lane & 1) ? TX : -1;\n"
+       "    const int hx = x0 + px, hy = y0 - 2 + hr;\n"
+       "    const bool yin = yrow >= dom.local_min[1] && yrow < dom.local_max[1];\n"
+       "    const bool hin = hy >= dom.local_min[1] && hy < dom.local_max[1] &&\n"
+       "                     hx >= dom.local_min[0] && hx < dom.local_max[0];\n"
+       "    bool xin_all = true;\n"
+       "#pragma unroll\n    for (int i = 0; i < VX; ++i)\n"
+       "      xin_all = xin_all && xs + i >= dom.local_min[0] && xs + i < dom.local_max[0];\n"
+       "    const auto zin = [&](int z) { return z >= dom.local_min[2] && z < dom.local_max[2]; };\n"
+    << "    const physis_hip::BufWin<" << T << "> aw = physis_hip::BufWin<" << T
+    << ">::Make(src.p, n, (long)(zb - 3) * nxy, (long)(ze + 3) * nxy);\n"
+    << "    const physis_hip::BufWin<" << T << "> dw = physis_hip::BufWin<" << T
+    << ">::Make(dst.p, n, (long)zb * nxy, (long)ze * nxy);\n"
+    << "    const physis_hip::BufWin<" << T << "> bw = physis_hip::BufWin<" << T
+    << ">::Make(bg.p, n, (long)(zb - 1) * nxy, (long)(ze + 1) * nxy);\n"
+    << "    const physis_hip::BufWin<" << T << "> mw = physis_hip::BufWin<" << T
+    << ">::Make(bg.p, n, (long)zb * nxy, (long)ze * nxy);\n";
+  for (auto *g : direct)
+    o << "    const physis_hip::BufWin<" << g->elem << "> " << g->name
+      << "_w = physis_hip::BufWin<" << g->elem << ">::Make(" << g->name
+      << ".p, n, (long)(zb - 1) * nxy, (long)(ze + 1) * nxy);\n";
+  o << "    SA st[PD];\n"
+       "    const typename SA::Addr ra = SA::Prepare(src.dim, x0, yrow, lane);\n"
+       "#pragma unroll\n    for (int k = 0; k < 3; ++k) {\n"
+       "      st[0].Load(aw, src.p, src.dim, ra, zb - 2 + k);\n"
+       "      st[0].Store(a_lds + k * SA::kPlane, wave, lane);\n    }\n"
+       "#pragma unroll\n    for (int k = 0; k < PD; ++k)\n"
+       "      st[k].Load(aw, src.p, src.dim, ra, zb + 1 + k);\n";
+  // centre values of u' (one step ahead) and of u''
+  for (auto *g : direct)
+    o << "    " << g->elem << " " << g->name << "_c1[2][VX], " << g->name
+      << "_c2[VX];\n";
+  o << "    " << T << " bg_c1[2][VX];\n";
+  o << "    // loads of plane z for u': the lane's VX points on mid rows, its halo\n"
+       "    // point in wave 0; emitted-grid values only where a point lies\n"
+       "    // outside the domain\n"
+       "    auto load1 = [&](int z, int slot) {\n"
+       "      if (wave == 0) {\n"
+       "        const int o = hlane ? hx + hy * nx + z * nxy : physis_hip::kDrop;\n"
+       "        const int ob = hlane && !(hin && zin(z)) ? o : physis_hip::kDrop;\n";
+  for (auto *g : direct)
+    o << "        " << g->name << "_w.template Load<1>(o, *reinterpret_cast<"
+      << g->elem << " (*)[1]>(&" << g->name << "_c1[slot][0]));\n";
+  o << "        bw.template Load<1>(ob, *reinterpret_cast<" << T
+    << " (*)[1]>(&bg_c1[slot][0]));\n"
+       "      } else {\n"
+       "        const int o = mid_row ? xs + yrow * nx + z * nxy : physis_hip::kDrop;\n"
+       "        const int ob = mid_row && !(yin && xin_all && zin(z)) ? o : physis_hip::kDrop;\n";
+  for (auto *g : direct)
+    o << "        " << g->name << "_w.template Load<VX>(o, " << g->name
+      << "_c1[slot]);\n";
+  o << "        bw.template Load<VX>(ob, bg_c1[slot]);\n      }\n    };\n"
+       "    load1(zb - 1, 0);\n"
+       "    __syncthreads();\n";
+  o << "    __PSCtx_" << n << " ctx;\n";
+  for (auto &g : k.grids) {
+    std::string src_view = &g == &gi ? "src" : (&g == &go ? "dst" : g.name);
+    o << "    ctx." << g.name << "_p = " << src_view << ".p;\n"
+      << "    for (int d = 0; d < 3; ++d) ctx." << g.name << "_dim[d] = "
+      << src_view << ".dim[d];\n";
+  }
+  o << "    for (int sc = zb - 1; sc <= ze + 1; sc += 4) {\n#pragma unroll\n"
+       "      for (int k = 0; k < 4; ++k) {\n"
+       "        const int s = sc + k;\n"
+       "        const int abase = k, bbase = k;\n"
+       "        const int cur = k & 1, nxt = cur ^ 1;\n"
+       "        if (s > ze + 1) break;  // uniform over the block\n"
+       "        load1(s + 1, nxt);\n"
+       "        const int o2 = out_row && s - 2 >= zb ? xs + yrow * nx + (s - 2) * nxy : physis_hip::kDrop;\n";
+  for (auto *g : direct)
+    o << "        " << g->name << "_w.template Load<VX, physis_hip::kStreamLoad>(o2, "
+      << g->name << "_c2);\n";
+  // u' on the lane's points
+  o << "        if (mid_row && s <= ze) {\n";
+  o << read_rows("a", "a_lds", "SA", "abase", "          ");
+  o << "          " << T << " uo[VX];\n          unsigned m = 0;\n"
+       "          const bool own = out_row && s >= zb && s < ze;\n"
+       "#pragma unroll\n          for (int i = 0; i < VX; ++i) {\n";
+  o << fill("a", "            ");
+  for (auto *g : direct)
+    o << "            ctx." << g->name << "_c = " << g->name << "_c1[cur][i];\n";
+  o << "            ctx." << O << "_e = false;\n"
+    << "            const bool in = yin && xs + i >= dom.local_min[0] && "
+       "xs + i < dom.local_max[0] && zin(s);\n";
+  if (red)
+    o << "            const " << RT << " r = " << n << "(xs + i, yrow, s, ctx" << fa
+      << ");\n            if constexpr (RED >= 0) acc1 = in && own ? " << ACC
+      << "::apply(acc1, r) : acc1;\n";
+  else
+    o << "            " << n << "(xs + i, yrow, s, ctx" << fa << ");\n";
+  o << "            uo[i] = in ? ctx." << O << "_v : bg_c1[cur][i];\n"
+       "            m |= in ? 1u << i : 0u;\n          }\n";
+  o << "          " << T << " *brow = b_lds + physis_hip::RingSlot<4>(bbase, 3) * "
+       "SB::kPlane + wave * SB::kRow + SB::kPad + lane * VX;\n"
+       "#pragma unroll\n          for (int i = 0; i < VX; ++i) brow[i] = uo[i];\n"
+       "          if (store_mid && own)\n"
+       "            physis_hip::StoreMasked<" << T << ", VX>(mw, "
+       "xs + yrow * nx + s * nxy, uo, m);\n"
+       "        }\n";
+  // u' halo columns
+  o << "        if (hlane && s <= ze) {\n";
+  for (size_t i = 0; i < gi.offsets.size(); ++i) {
+    const auto &off = gi.offsets[i];
+    o << "          ctx." << I << "[" << i << "] = a_lds[physis_hip::RingSlot<4>(abase, "
+      << off[2] + 1 << ") * SA::kPlane + (hr + " << off[1]
+      << ") * SA::kRow + SA::kPad + px + " << off[0] << "];\n";
+  }
+  for (auto *g : direct)
+    o << "          ctx." << g->name << "_c = " << g->name << "_c1[cur][0];\n";
+  o << "          ctx." << O << "_e = false;\n"
+    << "          " << (red ? "(void)" : "") << n << "(hx, hy, s, ctx" << fa << ");\n"
+    << "          b_lds[physis_hip::RingSlot<4>(bbase, 3) * SB::kPlane + hr * SB::kRow + "
+       "SB::kPad + px] = hin && zin(s) ? ctx." << O << "_v : bg_c1[cur][0];\n"
+       "        }\n";
+  // u'' on the tile
+  o << "        if (out_row && s - 2 >= zb) {\n";
+  o << read_rows("b", "b_lds", "SB", "bbase", "          ");
+  o << "          " << T << " uu[VX];\n          unsigned m = 0;\n"
+       "#pragma unroll\n          for (int i = 0; i < VX; ++i) {\n";
+  o << fill("b", "            ");
+  for (auto *g : direct)
+    o << "            ctx." << g->name << "_c = " << g->name << "_c2[i];\n";
+  o << "            ctx." << O << "_e = false;\n"
+    << "            const bool in = xs + i >= dom.local_min[0] && xs + i < dom.local_max[0];\n";
+  if (red)
+    o << "            const " << RT << " r = " << n << "(xs + i, yrow, s - 2, ctx" << fa
+      << ");\n            if constexpr (RED >= 0) acc2 = in ? " << ACC
+      << "::apply(acc2, r) : acc2;\n";
+  else
+    o << "            " << n << "(xs + i, yrow, s - 2, ctx" << fa << ");\n";
+  o << "            uu[i] = ctx." << O << "_v;\n"
+       "            m |= in ? 1u << i : 0u;\n          }\n"
+       "          physis_hip::StoreMasked<" << T
+    << ", VX>(dw, xs + yrow * nx + (s - 2) * nxy, uu, m);\n        }\n"
+       "        st[k % PD].Store(a_lds + physis_hip::RingSlot<4>(abase, 3) * SA::kPlane, "
+       "wave, lane);\n"
+       "        __syncthreads();\n"
+       "        st[k % PD].Load(aw, src.p, src.dim, ra, s + 2 + PD);\n"
+       "      }\n    }\n  }\n";
+  o << "  if constexpr (RED >= 0) {\n"
+    << "    __shared__ " << RT << " lds_r1[BY], lds_r2[BY];\n"
+    << "    const " << RT << " r1 = physis_hip::BlockReduce<" << RT
+    << ", RED < 0 ? 0 : RED, 64 * BY>(acc1, lds_r1);\n"
+    << "    const " << RT << " r2 = physis_hip::BlockReduce<" << RT
+    << ", RED < 0 ? 0 : RED, 64 * BY>(acc2, lds_r2);\n"
+       "    if (threadIdx.x == 0) {\n"
+       "      if (red1) red1[blockIdx.x] = r1;\n"
+       "      if (red2) red2[blockIdx.x] = r2;\n    }\n"
+       "  } else {\n    (void)acc1; (void)acc2; (void)red1; (void)red2;\n  }\n}\n\n";
+  // launcher, block count and run-time check.  Every centre-only grid
+  // holds 2 * VX values of u' in flight and VX of u'' per lane.  Without
+  // such grids the kernel is as light as the whole-grid one (16-wave
+  // blocks); with many (the 19-point Himeno kernel's 12), two points per
+  // lane and 12-wave blocks stay within the register file at three waves
+  // per SIMD (measured at 512x256x256: 107 Gpts/s with the per-sweep
+  // residual, against 107 for VX = 4 in 8-wave blocks and 91 for single
+  // sweeps).
+  const int kVX = direct.size() > 4 ? 2 : 4;
+  const int kBY = direct.empty() ? 16 : (direct.size() > 4 ? 12 : 8);
+  o << "static constexpr int __PS2VX_" << n << " = " << kVX << ", __PS2BY_" << n
+    << " = " << kBY << ";\n";
+  std::vector<std::string> wgrids = {"s." + I, "s." + O};
+  for (auto *g : direct) wgrids.push_back("s." + g->name);
+  o << "template <int VX, int BY>\nstatic physis_hip::RunShape __PSShape2_" << n
+    << "(const struct __PSStencil_" << n << " &s) {\n"
+    << "  static int per_cu = physis_hip::BlocksPerCU(\n"
+    << "      (const void *)__PSStencilRun2_" << n << "<VX, BY, -1>, 64 * BY);\n"
+    << "  return physis_hip::PlanRun(s.dom, 64 * VX, BY - 4, per_cu,\n"
+    << "      physis_hip::WindowChunkLimit({" << Join(wgrids, ", ") << "}, 3));\n}\n\n";
+  o << "static int __PSBlocks2_" << n << "(const struct __PSStencil_" << n
+    << " &s) {\n  return __PSShape2_" << n << "<__PS2VX_" << n << ", __PS2BY_"
+    << n << ">(s).blocks;\n}\n\n";
+  std::vector<std::string> largs = {"s.dom", "src", "dst",
+                                    "physis_hip::DevView<" + V + ">(s." + O + ")",
+                                    "store_mid"};
+  for (auto *g : direct)
+    largs.push_back("physis_hip::DevView<" + g->dev_view + ">(s." + g->name + ")");
+  for (auto &a : sa) largs.push_back("s." + a);
+  largs.push_back("sh");
+  largs.push_back("(" + RT + " *)red1");
+  largs.push_back("(" + RT + " *)red2");
+  const std::string LA = Join(largs, ", ");
+  o << "static void __PSLaunch2_" << n << "(const struct __PSStencil_" << n
+    << " &s, const " << V << " &src, const " << V
+    << " &dst, int store_mid, hipStream_t stream, void *red1, void *red2) {\n"
+    << "  constexpr int VX = __PS2VX_" << n << ", BY = __PS2BY_" << n << ";\n"
+    << "  const physis_hip::RunShape sh = __PSShape2_" << n << "<VX, BY>(s);\n"
+    << "  if (sh.blocks == 0) return;\n"
+    << "  const dim3 grid(sh.blocks), block(64 * BY);\n";
+  if (red && k.used_reduce) {
+    o << "  switch (red1 || red2 ? s.__reduce_op : -1) {\n";
+    for (const char *op : {"PS_MAX", "PS_MIN", "PS_SUM", "PS_PROD"})
+      o << "    case " << op << ":\n      hipLaunchKernelGGL((__PSStencilRun2_" << n
+        << "<VX, BY, " << op << ">), grid, block, 0, stream, " << LA
+        << ");\n      break;\n";
+    o << "    default:\n";
+  } else {
+    o << "  {\n";
+  }
+  o << "      hipLaunchKernelGGL((__PSStencilRun2_" << n
+    << "<VX, BY, -1>), grid, block, 0, stream, " << LA << ");\n  }\n}\n\n";
+  o << "// The pair (s0, s1) is " << n << "(a -> b), " << n
+    << "(b -> a) over one domain strictly\n// inside the grids, with the same "
+       "centre-only grids (neither a nor b) and\n// scalars: it may run as "
+       "two-sweep passes.\n";
+  o << "static bool __PSFusable2_" << n << "(const struct __PSStencil_" << n
+    << " &s0, const struct __PSStencil_" << n << " &s1) {\n"
+    << "  static const bool off = getenv(\"PHYSIS_HIP_NO_FUSE\") != NULL;\n"
+    << "  const __PSGrid *a = s0." << I << ", *b = s0." << O << ";\n"
+    << "  if (off || !a || !b || a == b || s1." << I << " != b || s1." << O
+    << " != a) return false;\n"
+    << "  if (a->num_dims != 3 || b->num_dims != 3 || a->dim[0] % 4) return false;\n"
+    << "  for (int d = 0; d < 3; ++d) {\n"
+       "    if (a->dim[d] != b->dim[d]) return false;\n"
+       "    if (s0.dom.local_min[d] != s1.dom.local_min[d] ||\n"
+       "        s0.dom.local_max[d] != s1.dom.local_max[d]) return false;\n"
+       "    if (s0.dom.local_min[d] < 1 || s0.dom.local_max[d] > a->dim[d] - 1 ||\n"
+       "        s0.dom.local_min[d] >= s0.dom.local_max[d]) return false;\n"
+       "  }\n";
+  for (auto *g : direct) {
+    const std::string G = g->name;
+    o << "  if (s0." << G << " != s1." << G << " || !s0." << G << " || s0." << G
+      << " == a || s0." << G << " == b || s0." << G << "->num_dims != 3) return false;\n"
+      << "  for (int d = 0; d < 3; ++d)\n    if (s0." << G
+      << "->dim[d] != a->dim[d]) return false;\n";
+  }
+  for (auto &a : sa)
+    o << "  if (__builtin_memcmp(&s0." << a << ", &s1." << a << ", sizeof(s0." << a
+      << "))) return false;\n";
+  if (red)
+    o << "  if ((s0.__reduce_out == NULL) != (s1.__reduce_out == NULL) ||\n"
+         "      (s0.__reduce_out && s0.__reduce_op != s1.__reduce_op)) return false;\n";
+  o << "  return true;\n}\n\n";
+  return o.str();
 }
 
 std::string Translator::EmitRunFunction(const RunSite &r) {
@@ -1936,50 +2378,29 @@ std::string Translator::EmitRunFunction(const RunSite &r) {
   for (size_t i = 0; i < r.kernels.size(); ++i)
     any_red = any_red || kernels_.at(r.kernels[i]).ret_type != "void";
   if (any_red) o << "  physis_hip::ReduceSlots __slots;\n";
+  // a fusable pair decides up front whether it runs as two-sweep passes:
+  // the partials of its reducing maps then come from the fused launches
+  const bool fused_pair = r.kernels.size() == 2 &&
+                          r.kernels[0] == r.kernels[1] &&
+                          kernels_.at(r.kernels[0]).fusable;
+  if (fused_pair)
+    o << "  const bool __fuse = iter >= 2 && __PSFusable2_" << r.kernels[0]
+      << "(s0, s1);\n";
   for (size_t i = 0; i < r.kernels.size(); ++i) {
     const KernelInfo &k = kernels_.at(r.kernels[i]);
     std::string si = "s" + std::to_string(i);
-    if (k.ret_type != "void")
-      o << "  physis_hip::MapReducer red" << i << "(" << si << ".__reduce_out, "
-        << si << ".__reduce_op, " << PsTypeOf(k.ret_type) << ",\n      " << si
-        << ".__reduce_out ? __PSBlocks_" << k.name << "(" << si
-        << ") : 0, iter, __slots);\n";
+    if (k.ret_type == "void") continue;
+    std::string blocks = "__PSBlocks_" + k.name + "(" + si + ")";
+    if (fused_pair && k.fuse_domain)
+      blocks = "(__fuse ? __PSBlocks2_" + k.name + "(" + si + ") : " + blocks +
+               ")";
+    o << "  physis_hip::MapReducer red" << i << "(" << si << ".__reduce_out, "
+      << si << ".__reduce_op, " << PsTypeOf(k.ret_type) << ",\n      " << si
+      << ".__reduce_out ? " << blocks << " : 0, iter, __slots);\n";
   }
   const bool fuse = r.kernels.size() == 2 && r.kernels[0] == r.kernels[1] &&
                     kernels_.at(r.kernels[0]).fusable;
-  if (fuse) {
-    // two-sweep passes: an odd count starts with one plain pair; the passes
-    // ping-pong between a and a scratch grid, ending in a, and the last one
-    // also writes its intermediate sweep to b, so a and b end as the plain
-    // pairs leave them
-    const KernelInfo &k = kernels_.at(r.kernels[0]);
-    const GridParam &gi = k.grids[k.fuse_in], &go = k.grids[k.fuse_out];
-    // Each pass is one pair.  Even counts ping-pong a -> t -> a ...; odd
-    // counts (>= 3) send the first pass to b instead, a -> b -> t -> a ...
-    // (b is free until the last pass writes it).  Either way the last pass
-    // reads t and writes a, and its intermediate sweep to b.
-    o << "  int i0 = 0;\n"
-      << "  if (iter >= 2 && __PSFusable2_" << k.name << "(s0, s1)) {\n"
-      << "    physis_hip::CheckGrid3D(s0." << gi.name << ", " << gi.ps_type << ");\n"
-      << "    physis_hip::CheckGrid3D(s0." << go.name << ", " << go.ps_type << ");\n"
-      << "    " << gi.dev_view << " va = physis_hip::DevView<" << gi.dev_view
-      << ">(s0." << gi.name << ");\n"
-      << "    " << gi.dev_view << " vb = physis_hip::DevView<" << gi.dev_view
-      << ">(s0." << go.name << ");\n"
-      << "    " << gi.dev_view << " vt = va;\n"
-      << "    vt.p = (" << gi.elem << " *)__PSHipScratch(8, (size_t)s0." << gi.name
-      << "->num_elms * sizeof(" << gi.elem << "));\n"
-      << "    " << gi.dev_view << " src = va;\n"
-      << "    int j = 0;\n"
-      << "    if (iter % 2) {\n"
-      << "      __PSLaunch2_" << k.name << "(s0, va, vb, NULL, stream);\n"
-      << "      src = vb;\n      j = 1;\n    }\n"
-      << "    for (; j < iter; ++j) {\n"
-      << "      const " << gi.dev_view << " dst = ((iter - 1 - j) % 2 == 0) ? va : vt;\n"
-      << "      __PSLaunch2_" << k.name << "(s0, src, dst, j == iter - 1 ? vb.p : NULL, stream);\n"
-      << "      src = dst;\n    }\n"
-      << "    i0 = iter;\n  }\n";
-  }
+  if (fuse) o << EmitFusedRun(kernels_.at(r.kernels[0]));
   o << "  for (int i = " << (fuse ? "i0" : "0") << "; i < iter; ++i) {\n";
   for (size_t i = 0; i < r.kernels.size(); ++i) {
     const KernelInfo &k = kernels_.at(r.kernels[i]);

@@ -84,6 +84,9 @@ struct KernelInfo {
   std::string rb_note;  // whether the in-place colour sweep matches REF order
   // two-sweep fusion (temporal blocking) of map pairs k(a->b), k(b->a)
   bool fusable = false;
+  // the pair fuses over a domain strictly inside the grid (points outside
+  // it keep their values), rather than over the whole grid
+  bool fuse_domain = false;
   int fuse_in = -1, fuse_out = -1;  // grid indices of the read and emitted grids
   std::string fuse_why;             // why not, for the analysis comment
   // the body compares index parameter y / z (axis 1 / 2) with something:
@@ -137,6 +140,9 @@ class Translator {
   std::string EmitGenericRunKernel(const KernelInfo &k);
   std::string EmitLauncher(const KernelInfo &k);
   std::string EmitFusedRunKernel(const KernelInfo &k);
+  std::string EmitFusedDomainKernel(const KernelInfo &k);
+  // the run-function branch that runs a fusable pair as two-sweep passes
+  std::string EmitFusedRun(const KernelInfo &k);
   // Interior split of a wave step at rows y, plane z: the wave-uniform
   // condition and the facts assumed under it ("" when the body compares
   // neither y nor z).

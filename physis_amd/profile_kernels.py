@@ -7,11 +7,15 @@ It issues, in order: a device-to-device copy of one 512 MiB grid (the
 runtime's plain copy kernel, whose FETCH_SIZE / WRITE_SIZE calibrate the
 counters' units), at 512^3 four two-sweep launches of the 7-point clamped
 kernel, five single sweeps of it, 10 periodic sweeps (five two-sweep
-launches), 3 PSReduce SUMs at 512^3, 10 Himeno sweeps at 512x256x256, and
-the precision tier at 512^3: 10 BF16 inner sweeps and 3 fused FP64
-update + residual passes.
+launches), 3 PSReduce SUMs at 512^3, 10 Himeno sweeps at 512x256x256 (five
+two-sweep launches) and three pairs of single Himeno sweeps, and the
+precision tier at 512^3: 10 BF16 inner sweeps (one pair of single sweeps,
+four two-sweep launches) and 3 fused FP64 update + residual passes.
 Then `python3 -m physis_amd.profile_summary OUT_fetch OUT_write` writes the
 per-kernel HBM traffic summary.
+
+`python3 -m physis_amd.profile_kernels diffusion` runs only the copy and the
+7-point launches (the workload of the instruction-counter profiles).
 """
 from __future__ import annotations
 
@@ -26,7 +30,7 @@ from .programs import Diffusion7, Himeno
 COPY_BYTES = 512 * 2 ** 20
 
 
-def main() -> None:
+def main(which: str = "all") -> None:
     ps.PSInit()
     hip = C.CDLL("libamdhip64.so")
     hip.hipMemcpyDtoD.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
@@ -48,6 +52,10 @@ def main() -> None:
         ps.PSReduce(ps.PS_SUM, g1)
     ps.PSGridFree(g1)
     ps.PSGridFree(g2)
+    if which == "diffusion":
+        ps.PSFinalize()
+        print("profile workload done (diffusion)")
+        return
     hs = (512, 256, 256)
     coefs, p = himeno_inputs(*hs, seed=0)
     grids = []
@@ -55,7 +63,10 @@ def main() -> None:
         g = ps.PSGridNew(hs)
         ps.PSGridCopyin(g, c)
         grids.append(g)
-    Himeno().run(10, grids[12], grids[13], grids[:12])
+    him = Himeno()
+    him.run(10, grids[12], grids[13], grids[:12])   # five two-sweep passes
+    for _ in range(3):                              # single-sweep pairs
+        him.run(2, grids[12], grids[13], grids[:12])
     for g in grids:
         ps.PSGridFree(g)
     # precision tier at 512^3: 10 BF16 inner sweeps, 3 fused FP64
@@ -84,4 +95,5 @@ def main() -> None:
 
 
 if __name__ == "__main__":
-    main()
+    import sys
+    main(sys.argv[1] if len(sys.argv) > 1 else "all")

@@ -28,16 +28,23 @@ from collections import defaultdict
 
 COPY_BYTES = 512 * 2 ** 20
 KERNELS = [  # (label, name fragment, algorithmic bytes per launch)
-    # two-sweep kernels: algorithmic bytes of the two sweeps a launch does
-    ("7-pt clamped two-sweep kernel, 512^3 (2 sweeps per launch)",
-     "__PSStencilRun2_diffusion7_kernel<", 2 * 8 * 512 ** 3),
+    # Two-sweep kernels: the minimum HBM traffic of a launch, which reads
+    # its inputs once and writes its output once for two sweeps.
+    ("7-pt clamped two-sweep kernel, 512^3 (2 sweeps per launch, "
+     "minimum bytes: read u, write u'')",
+     "__PSStencilRun2_diffusion7_kernel<", 8 * 512 ** 3),
     ("7-pt clamped run kernel, 512^3", "__PSStencilRun_diffusion7_kernel<",
      8 * 512 ** 3),
-    ("7-pt periodic two-sweep kernel, 512^3 (2 sweeps per launch)",
-     "__PSStencilRun2_diffusion7_periodic_kernel<", 2 * 8 * 512 ** 3),
+    ("7-pt periodic two-sweep kernel, 512^3 (2 sweeps per launch, minimum "
+     "bytes)", "__PSStencilRun2_diffusion7_periodic_kernel<", 8 * 512 ** 3),
+    ("19-pt Himeno two-sweep kernel, 512x256x256 (2 sweeps per launch, "
+     "minimum bytes: 12 fields + p read, p'' written)",
+     "__PSStencilRun2_jacobi_kernel<", 56 * 512 * 256 * 256),
     ("19-pt Himeno run kernel, 512x256x256", "__PSStencilRun_jacobi_kernel<",
      56 * 512 * 256 * 256),
     ("PSReduce SUM stage 1, 512^3", "ReduceStage1<float, 2>", 4 * 512 ** 3),
+    ("BF16 two-sweep inner Jacobi pass, 512^3 (2 sweeps per launch, "
+     "minimum bytes)", "TwoSweepKernel<physis_hip::bf16_t", 28 * 512 ** 3),
     ("BF16 inner Jacobi sweep, 512^3", "SweepKernel<physis_hip::bf16_t",
      28 * 512 ** 3),
     ("FP64 update + residual (BF16 E/RHS), 512^3",
@@ -55,12 +62,20 @@ def _counter(d: str, name: str):
     return vals
 
 
+def _matches(name: str, frag: str) -> bool:
+    """frag names the kernel itself (it starts the name or follows a space or
+    a namespace separator), not a longer name that ends with it."""
+    import re
+    return re.search(r"(^|[\s:])" + re.escape(frag), name) is not None
+
+
 def _pick(vals, frag):
-    ks = [k for k in vals if frag in k]
-    if len(ks) != 1:
-        raise SystemExit(f"kernel {frag!r}: {len(ks)} matches")
-    v = vals[ks[0]]
-    return ks[0], sum(v) / len(v), len(v)
+    ks = [k for k in vals if _matches(k, frag)]
+    if not ks:
+        raise SystemExit(f"kernel {frag!r}: no match")
+    k = max(ks, key=lambda k: len(vals[k]))
+    v = vals[k]
+    return k, sum(v) / len(v), len(v)
 
 
 def main(fetch_dir, write_dir, kt_dir, out_md) -> None:
@@ -81,9 +96,10 @@ def main(fetch_dir, write_dir, kt_dir, out_md) -> None:
         name, f, n = _pick(fetch, frag)
         _, w, _ = _pick(write, frag)
         fb, wb = f * fs, w * ws
-        cands = [(k, v) for k, v in kt.items() if frag in k]
-        if len(cands) > 1:  # the plain (non-reducing) VX = 4 instantiation
-            cands = [(k, v) for k, v in cands if "<4" in k and "-1>" in k]
+        cands = [(k, v) for k, v in kt.items() if _matches(k, frag)]
+        # several instantiations (vector width, reduction operator): the
+        # one launched most often
+        cands.sort(key=lambda kv: -kv[1][1])
         avg_ns = cands[0][1][0] if cands else float("nan")
         rows.append((label, fb, wb, alg, avg_ns, n))
         if "diffusion7_kernel<" in frag:

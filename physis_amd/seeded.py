@@ -8,6 +8,9 @@ restates the same generator in C.
 """
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 _GAMMA = np.uint64(0x9E3779B97F4A7C15)
@@ -21,17 +24,33 @@ def _mix(z: np.ndarray) -> np.ndarray:
     return z ^ (z >> np.uint64(31))
 
 
+def _threads() -> int:
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
 def uniform01(n: int, seed: int = 0, start: int = 0,
-              chunk: int = 1 << 24) -> np.ndarray:
-    """float32[n]: elements start .. start+n-1 of the stream for `seed`."""
+              chunk: int = 1 << 22) -> np.ndarray:
+    """float32[n]: elements start .. start+n-1 of the stream for `seed`.
+    Chunks are filled by a thread pool (numpy releases the GIL); every
+    element depends only on its index, so the result is the same for any
+    chunking."""
     out = np.empty(n, dtype=np.float32)
     scale = np.float32(1.0 / 16777216.0)
-    with np.errstate(over="ignore"):
-        for lo in range(0, n, chunk):
-            hi = min(n, lo + chunk)
+
+    def fill(lo: int) -> None:
+        hi = min(n, lo + chunk)
+        with np.errstate(over="ignore"):
             idx = np.arange(start + lo + 1, start + hi + 1, dtype=np.uint64)
             z = np.uint64(seed) + idx * _GAMMA
             out[lo:hi] = (_mix(z) >> np.uint64(40)).astype(np.float32) * scale
+
+    starts = range(0, n, chunk)
+    if n <= chunk:
+        for lo in starts:
+            fill(lo)
+    else:
+        with ThreadPoolExecutor(_threads()) as ex:
+            list(ex.map(fill, starts))
     return out
 
 

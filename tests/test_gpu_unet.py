@@ -147,7 +147,8 @@ def test_network_loss_and_gradient(ps):
         loss, grad = net.loss_grad(rhs, bnd, tg, 0, nb)
         rloss, rgrad = U.loss_grad(params, coefs[11], coefs[10], target,
                                    (nx, ny, nzs), nb, scale)
-        print("loss", loss, rloss, flush=True)
+        print("loss", loss, rloss, "rel", abs(loss - rloss) / rloss,
+              flush=True)
         assert abs(loss - rloss) <= 1e-2 * rloss
         err = np.linalg.norm(grad - rgrad) / np.linalg.norm(rgrad)
         cos = float(grad @ rgrad / np.linalg.norm(grad) / np.linalg.norm(rgrad))
@@ -158,7 +159,9 @@ def test_network_loss_and_gradient(ps):
         for ci, co in U.LAYERS:
             n = co * 27 * ci + co
             g, r = grad[off:off + n], rgrad[off:off + n]
-            assert np.linalg.norm(g - r) <= 0.1 * np.linalg.norm(r), (ci, co)
+            lerr = np.linalg.norm(g - r) / np.linalg.norm(r)
+            print(f"layer {ci}->{co}: grad rel err {lerr:.3e}", flush=True)
+            assert lerr <= 0.1, (ci, co)
             off += n
         # prediction = out_scale * out * bnd
         pg = ps.PSGridNew(shape)
