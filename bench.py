@@ -302,6 +302,107 @@ def precision_tier(ps, hip) -> dict:
     return out
 
 
+def warm_start_leg(ps) -> dict:
+    """The learned warm start at its own workload: 64 Poisson problems of
+    128^3 (face coefficients with 5% noise and a3 balancing them, 4-24
+    Gaussians with sigma 2-16 cells), solved to relative residual 1e-6
+    (FP64 residual on the GPU) by the BF16 refinement from zero and from the
+    U-Net's prediction, against plain FP32 Jacobi from zero, which stops
+    above 1e-6 (its stagnation rule).  The network's weights come from
+    physis_amd/warmstart_train.py (physis_amd/weights/unet128.npz; its
+    training cost is in unet128.json); the 64 problems are not among its
+    training problems."""
+    from physis_amd.configs import poisson_problem
+    from physis_amd.poisson import PoissonSolver, time_to
+    from physis_amd.warmstart import UNet
+    wpath = os.path.join(ROOT, "physis_amd", "weights", "unet128.npz")
+    if not os.path.exists(wpath):
+        return {"skipped": "no trained weights (run python -m "
+                           "physis_amd.warmstart_train)"}
+    w = np.load(wpath)
+    with open(os.path.splitext(wpath)[0] + ".json") as f:
+        meta = json.load(f)
+    n, nb, tol = 128, 64, 1e-6
+    m = n ** 3
+    log("generating 64 problems of 128^3 for the warm start")
+    coefs = poisson_problem(n, n, n, nbatch=nb, noise=0.05, gaussians=(4, 24),
+                            sigma=(2.0, 16.0), seed=7, consistent=True)
+    shape = (n, n, n * nb)
+    grids = []
+    for c in coefs:
+        g = ps.PSGridNew(shape)
+        ps.PSGridCopyin(g, c)
+        grids.append(g)
+    del coefs
+    H = 8192
+    runs = {}
+    log("plain FP32 Jacobi from zero")
+    s = PoissonSolver(grids, nb, "fp32")
+    p = ps.PSGridNew(shape)
+    runs["plain_fp32_zero"] = s.solve_plain(p, tol=tol, check=500, history=H)
+    s.free()
+    ps.PSGridFree(p)
+    log("BF16 refinement from zero")
+    s = PoissonSolver(grids, nb, "bf16")
+    p = ps.PSGridNew(shape, np.float64)
+    runs["mixed_bf16_zero"] = s.solve_mixed(p, tol=tol, inner=200, history=H)
+    log("BF16 refinement from the U-Net prediction")
+    net = UNet(n, n, n, 8, float(w["out_scale"]), seed=0)
+    net.set_params(w["params"])
+    t0 = time.perf_counter()
+    for f0 in range(0, nb, 8):
+        net.predict(grids[11], grids[10], f0, 8, p)
+    t_pred = time.perf_counter() - t0
+    st = s.solve_mixed(p, tol=tol, inner=200, history=H)
+    st["predict_seconds"] = round(t_pred, 4)
+    runs["mixed_bf16_warm"] = st
+    s.free()
+    net.free()
+    ps.PSGridFree(p)
+    for g in grids:
+        ps.PSGridFree(g)
+    floor = max(runs["plain_fp32_zero"]["rel_residual"], tol)
+    out = {"problems": nb, "n": n, "tol": tol,
+           "plain_fp32_floor": runs["plain_fp32_zero"]["rel_residual"]}
+    at = {}
+    for k, v in runs.items():
+        hist = v.pop("history")
+        pre = v.get("predict_seconds", 0.0)
+        row = {kk: v[kk] for kk in ("converged", "stagnated", "sweeps",
+                                    "rel_residual", "seconds")}
+        row["initial_rel_residual"] = hist[0][1] if hist else None
+        for lv, name in ((floor, "to_plain_floor"), (tol, "to_1e-6")):
+            t = time_to(hist, lv)
+            if t is not None:
+                row[name] = {"sweeps": round(t[0]), "seconds":
+                             round(t[1] + pre, 3)}
+                at[(k, name)] = t[1] + pre
+        if pre:
+            row["predict_seconds"] = pre
+        out[k] = row
+    sp = {}
+    base = at.get(("plain_fp32_zero", "to_plain_floor"))
+    for k in ("mixed_bf16_zero", "mixed_bf16_warm"):
+        if base and (k, "to_plain_floor") in at:
+            sp[f"{k}_vs_plain_fp32_at_its_floor"] = round(
+                base / at[(k, "to_plain_floor")], 3)
+    if ("mixed_bf16_zero", "to_1e-6") in at and \
+            ("mixed_bf16_warm", "to_1e-6") in at:
+        sp["network_gain_to_1e-6"] = round(
+            at[("mixed_bf16_zero", "to_1e-6")] /
+            at[("mixed_bf16_warm", "to_1e-6")], 3)
+        sw0 = out["mixed_bf16_zero"]["to_1e-6"]["sweeps"]
+        sw1 = out["mixed_bf16_warm"]["to_1e-6"]["sweeps"]
+        sp["network_sweep_saving_to_1e-6"] = round(1 - sw1 / sw0, 4)
+    out["speedups"] = sp
+    out["training"] = {k: meta.get(k) for k in (
+        "ntrain_solved", "pairs", "steps", "batch", "solve_seconds",
+        "train_seconds", "loss_last")}
+    out["note"] = ("plain FP32 Jacobi never reaches 1e-6 (FP32 rounding of "
+                   "p); its speedups are taken at the residual it stops at")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,6 +412,8 @@ def main() -> int:
                     help="skip the precision-tier leg")
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline leg")
+    ap.add_argument("--no-warmstart", action="store_true",
+                    help="skip the learned warm-start leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: measures one GPU only; --gpus {args.gpus} is not "
@@ -509,6 +612,8 @@ def main() -> int:
     }
     if not args.no_poisson:
         line["precision_tier"] = precision_tier(ps, hip)
+    if not args.no_warmstart:
+        line["warm_start_64x128"] = warm_start_leg(ps)
     if not args.no_cpu:
         log("timing the CPU baselines")
         cpu = cpu_baselines(a, DIFFUSION_COEF)

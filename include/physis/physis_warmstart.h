@@ -69,6 +69,16 @@ double PSUNetTrainStep(__PSUNet *net, const __PSGrid *rhs,
                        const __PSGrid *bnd, const __PSGrid *target, int first,
                        int count, float lr);
 
+/* Training-set augmentation: problems [first, first + count) of the FP32
+ * grid src (problems of nx x ny x nzs stacked along z) copied to problems
+ * [dst_first, ...) of dst with the axes whose bit is set in flips (1 x,
+ * 2 y, 4 z) reversed.  The reflection of a solved problem (right-hand side
+ * and solution) is the solution of the reflected problem, whose
+ * coefficients (+-noise about the same values) are as likely as the
+ * original's.  src and dst must be different grids. */
+void PSUNetReflect(const __PSGrid *src, int first, int count, __PSGrid *dst,
+                   int dst_first, int nx, int ny, int nzs, int flips);
+
 /* ---- single layers on host arrays (parity tests).  Tensors are
  * channels-last BF16 bit patterns [n][d][h][w][c]; w is W[cout][27][cin]
  * as BF16 bit patterns, bias FP32. */

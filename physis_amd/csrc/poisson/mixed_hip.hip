@@ -617,10 +617,17 @@ void LaunchSweep(const __PSPoisson *s, const M *in, M *out, Fields<M> f,
 
 // Waves per block of the two-sweep kernel (TY = BY - 4 output rows) and its
 // points per lane.  The fields take 2 * VX values of E' in flight and VX of
-// E'' per lane: at VX = 4 an 8-wave block (two waves per SIMD, up to 256
-// VGPRs) holds them without spills for every storage type.
-template <typename M> struct TwoSweepBY { static constexpr int value = 8; };
-inline int TwoSweepVX(int nx) { return (nx % 4 == 0 && nx > 128) ? 4 : 2; }
+// E'' per lane.  Measured at 512^3 (ms per sweep; pairs of single sweeps:
+// 0.807 BF16, 1.448 FP32): 16-bit fields run best at VX = 2 in 12-wave
+// blocks (0.655; 130 VGPRs, three waves per SIMD; VX = 4 in 8-wave blocks
+// 0.827, in 12-wave blocks 0.732 with spills), FP32 fields at VX = 4 in
+// 8-wave blocks (0.984; VX = 2 in 12-wave blocks 1.104).
+template <typename M> struct TwoSweepBY { static constexpr int value = 12; };
+template <> struct TwoSweepBY<float> { static constexpr int value = 8; };
+template <typename M>
+int TwoSweepVX(int nx) { return 2; }
+template <>
+int TwoSweepVX<float>(int nx) { return (nx % 4 == 0 && nx > 128) ? 4 : 2; }
 
 template <typename M>
 const void *TwoSweepFn(int vx) {
@@ -873,7 +880,8 @@ __PSPoisson *PSPoissonNew(__PSGrid *const coef[12], int nbatch, int storage) {
       (const void *)ResidualKernel<double, bf16_t, 2, true, true>, kWave * kBY);
   s->sweep_plan = MakePlan(nx, ny, s->nzs, nbatch, s->sweep_vx, per_cu_sweep);
   {
-    s->sweep2_vx = TwoSweepVX(nx);
+    s->sweep2_vx = storage == PS_STORAGE_FP32 ? TwoSweepVX<float>(nx)
+                                              : TwoSweepVX<bf16_t>(nx);
     const void *fn = storage == PS_STORAGE_BF16   ? TwoSweepFn<bf16_t>(s->sweep2_vx)
                      : storage == PS_STORAGE_FP16 ? TwoSweepFn<fp16_t>(s->sweep2_vx)
                                                   : TwoSweepFn<float>(s->sweep2_vx);

@@ -161,6 +161,375 @@ __global__ __launch_bounds__(256) void ConvKernel(
   }
 }
 
+// 3x3x3 convolution marching z (the 2.5D scheme of the stencil run
+// kernels).  Block (x tile, y tile, problem n, z chunk) owns XT x TY output
+// positions of every plane of its chunk.  An LDS ring holds the input planes
+// z-1 .. z+1 over rows y0-1 .. y0+TY and x0-1 .. x0+XT (zero outside the
+// grid) and the plane z+2 being staged; the plane after that is loaded into
+// registers a step ahead.  Each input plane is thus staged once per tile
+// (the per-plane kernel above stages three), with one barrier per step.
+// The GEMM is D[cout][16 x] = W[cout][32 k] X[32 k][16 x]: the weights are
+// the A operand (registers, or LDS when they are many), the staged input the
+// B operand (one aligned 16-byte LDS read per lane), and a lane's result is
+// four consecutive channels of one position, stored as one 8-byte word.
+template <int CIN, int COUT, bool RELU, int XT, int TY>
+__global__ __launch_bounds__(64 * TY) void ConvZKernel(
+    const u16 *__restrict__ x, const u16 *__restrict__ wt,
+    const float *__restrict__ bias, u16 *__restrict__ y, int D, int H, int W,
+    int tz, int zchunk) {
+  constexpr int K = 27 * CIN, KB = (K + 31) / 32, KP = KB * 32;
+  constexpr int MT = (COUT + 15) / 16;
+  constexpr bool kRegW = KB * MT <= 14;
+  constexpr int XS = XT + 2, RS = TY + 2, NV = CIN / 8;
+  constexpr int PLANE = RS * XS * CIN;  // u16 elements of a staged plane
+  constexpr int SV = RS * XS * NV;      // 16-byte vectors of a staged plane
+  constexpr int NTH = 64 * TY;
+  constexpr int VPT = (SV + NTH - 1) / NTH;
+  constexpr int WL = kRegW ? 1 : MT * 16 * KP;  // weights staged in LDS
+  __shared__ __attribute__((aligned(16))) u16 ring[4 * PLANE];
+  __shared__ __attribute__((aligned(16))) u16 wlds[WL];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const int tiles_x = (W + XT - 1) / XT;
+  const int x0 = (blockIdx.x % tiles_x) * XT, y0 = (blockIdx.x / tiles_x) * TY;
+  const int n = blockIdx.y / tz;
+  const int zb = (blockIdx.y - n * tz) * zchunk;
+  const int ze = min(zb + zchunk, D);
+  if (zb >= ze) return;  // uniform over the block
+  auto fetch = [&](int zz, uint4 (&buf)[VPT]) {
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) {
+      const int v = threadIdx.x + i * NTH;
+      const int c8 = v % NV, t = v / NV;
+      const int xp = t % XS, pr = t / XS;
+      const int yy = y0 - 1 + pr, xx = x0 - 1 + xp;
+      uint4 u = make_uint4(0, 0, 0, 0);
+      if (v < SV && zz >= 0 && zz < D && yy >= 0 && yy < H && xx >= 0 &&
+          xx < W)
+        u = *reinterpret_cast<const uint4 *>(
+            x + ((((long)n * D + zz) * H + yy) * W + xx) * CIN + c8 * 8);
+      buf[i] = u;
+    }
+  };
+  auto put = [&](int slot, const uint4 (&buf)[VPT]) {
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) {
+      const int v = threadIdx.x + i * NTH;
+      if (v < SV)
+        *reinterpret_cast<uint4 *>(ring + slot * PLANE + (v / NV) * CIN +
+                                   (v % NV) * 8) = buf[i];
+    }
+  };
+  uint4 buf[VPT];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    fetch(zb - 1 + k, buf);
+    put(k, buf);
+  }
+  fetch(zb + 2, buf);
+  bf16x8 aw[kRegW ? KB : 1][MT];
+  if constexpr (kRegW) {
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+      for (int t = 0; t < MT; ++t) {
+        const int co = t * 16 + r;
+        aw[kb][t] = co < COUT ? Load8(wt + (long)co * KP + kb * 32 + 8 * g)
+                              : Zero8();
+      }
+  } else {
+    for (int v = threadIdx.x; v < MT * 16 * KP / 8; v += NTH) {
+      const int co = (v * 8) / KP;
+      uint4 u = make_uint4(0, 0, 0, 0);
+      if (co < COUT) u = *reinterpret_cast<const uint4 *>(wt + (long)v * 8);
+      *reinterpret_cast<uint4 *>(wlds + v * 8) = u;
+    }
+  }
+  // bias of the four channels co = 16 t + 4 g + v this lane receives
+  float bb[MT][4];
+#pragma unroll
+  for (int t = 0; t < MT; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int co = t * 16 + 4 * g + v;
+      bb[t][v] = (bias && co < COUT) ? bias[co] : 0.0f;
+    }
+  __syncthreads();
+  const int yy = y0 + wv;
+  for (int z = zb; z < ze; ++z) {
+    const int base = z - zb;  // ring slot of plane z - 1 (mod 4)
+    if (yy < H) {
+      const long row = (((long)n * D + z) * H + yy) * W;
+      for (int xs0 = 0; xs0 < XT && x0 + xs0 < W; xs0 += 16) {
+        f32x4 acc[MT];
+#pragma unroll
+        for (int t = 0; t < MT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const int xa = xs0 + r;
+#pragma unroll 4
+        for (int kb = 0; kb < KB; ++kb) {
+          const int k0 = kb * 32 + 8 * g;
+          bf16x8 b = Zero8();
+          if (k0 < K) {
+            const int tap = k0 / CIN, ci0 = k0 - tap * CIN;
+            const int slot = (base + tap / 9) & 3;
+            b = *reinterpret_cast<const bf16x8 *>(
+                ring + slot * PLANE +
+                ((wv + (tap / 3) % 3) * XS + xa + tap % 3) * CIN + ci0);
+          }
+#pragma unroll
+          for (int t = 0; t < MT; ++t) {
+            bf16x8 a;
+            if constexpr (kRegW)
+              a = aw[kb][t];
+            else
+              a = *reinterpret_cast<const bf16x8 *>(
+                  wlds + (t * 16 + r) * KP + kb * 32 + 8 * g);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0,
+                                                             0, 0);
+          }
+        }
+        const int xo = x0 + xs0 + r;
+        if (xo < W) {
+#pragma unroll
+          for (int t = 0; t < MT; ++t) {
+            const int co = t * 16 + 4 * g;
+            if (co >= COUT) continue;
+            unsigned w2[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              float v0 = acc[t][2 * h] + bb[t][2 * h];
+              float v1 = acc[t][2 * h + 1] + bb[t][2 * h + 1];
+              if (RELU) {
+                v0 = v0 > 0.0f ? v0 : 0.0f;
+                v1 = v1 > 0.0f ? v1 : 0.0f;
+              }
+              w2[h] = (unsigned)F2Bf(v0) | ((unsigned)F2Bf(v1) << 16);
+            }
+            *reinterpret_cast<uint2 *>(y + (row + xo) * COUT + co) =
+                make_uint2(w2[0], w2[1]);
+          }
+        }
+      }
+    }
+    put((base + 3) & 3, buf);
+    __syncthreads();
+    fetch(z + 3, buf);
+  }
+}
+
+// Filter gradient partials marching z: block (x tile, y tile, problem, z
+// chunk) sums over its XT x TY x zchunk positions
+//   part[block][co][k] = sum_p dz[p][co] x[p + off(k)][ci],  k = tap CIN + ci
+// as a GEMM over positions (K) on v_mfma_f32_16x16x32_bf16: A = dz^T (a
+// lane's 8 positions of one channel, staged transposed dzT[co][y][x]), B =
+// the shifted input (8 consecutive x of one channel, staged transposed
+// xT[plane][row][ci][x] in a ring of 4 planes, read as 32-bit words with a
+// funnel shift for odd offsets).  Wave w takes column tiles w, w + TY, ...
+// against the A operands of every 32-position step of the plane; wave 0
+// also sums dz per channel for the bias.  The planes of the input are
+// staged once per tile.
+template <int CIN, int COUT, int XT, int TY>
+__global__ __launch_bounds__(64 * TY) void WgradZKernel(
+    const u16 *__restrict__ x, const u16 *__restrict__ dz,
+    float *__restrict__ part, float *__restrict__ bpart, int D, int H, int W,
+    int tz, int zchunk) {
+  static_assert(XT % 32 == 0, "steps of 32 positions along x");
+  constexpr int K = 27 * CIN, NTN = (K + 15) / 16, MT = (COUT + 15) / 16;
+  constexpr int TPW = (NTN + TY - 1) / TY;  // column tiles per wave
+  constexpr int XR = XT + 2, RS = TY + 2;
+  constexpr int RW = CIN * XR / 2;          // 32-bit words per staged row
+  constexpr int PW = RS * RW;               // words per staged plane
+  constexpr int NTH = 64 * TY;
+  constexpr int XV = RS * XR * (CIN / 8);   // 16-byte input vectors per plane
+  constexpr int XVT = (XV + NTH - 1) / NTH;
+  constexpr int DV = TY * XT * (COUT / 8);  // 16-byte dz vectors per plane
+  constexpr int DVT = (DV + NTH - 1) / NTH;
+  static_assert(XR % 2 == 0, "x pairs");
+  __shared__ unsigned xT[4 * PW + 4];
+  __shared__ __attribute__((aligned(16))) u16 dzT[COUT * TY * XT];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const int tiles_x = (W + XT - 1) / XT;
+  const int x0 = (blockIdx.x % tiles_x) * XT, y0 = (blockIdx.x / tiles_x) * TY;
+  const int n = blockIdx.y / tz;
+  const int zb = (blockIdx.y - n * tz) * zchunk;
+  const int ze = min(zb + zchunk, D);
+  const long bid = (long)blockIdx.y * gridDim.x + blockIdx.x;
+  // word offset (within a plane) of each tile column's (row dy, ci, x dx)
+  int lofs[TPW], lpl[TPW];
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) {
+    const int col = (wv + TY * i) * 16 + r;
+    if (col < K) {
+      const int tap = col / CIN, ci = col - tap * CIN;
+      lpl[i] = tap / 9;
+      lofs[i] = (((tap / 3) % 3) * CIN + ci) * XR + tap % 3;  // elements
+    } else {
+      lpl[i] = 0;
+      lofs[i] = -1;
+    }
+  }
+  f32x4 acc[TPW][MT];
+#pragma unroll
+  for (int i = 0; i < TPW; ++i)
+#pragma unroll
+    for (int m = 0; m < MT; ++m) acc[i][m] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float bsum[MT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m) bsum[m] = 0.0f;
+  // input plane zz, rows y0-1 .. y0+TY, x pairs: vector v -> (row, x pair
+  // half, channel group)
+  auto fetchx = [&](int zz, uint4 (&buf)[XVT]) {
+#pragma unroll
+    for (int i = 0; i < XVT; ++i) {
+      const int v = threadIdx.x + i * NTH;
+      const int c8 = v % (CIN / 8), t = v / (CIN / 8);
+      const int xp = t % XR, row = t / XR;
+      const int yy = y0 - 1 + row, xx = x0 - 1 + xp;
+      uint4 u = make_uint4(0, 0, 0, 0);
+      if (v < XV && zz >= 0 && zz < D && yy >= 0 && yy < H && xx >= 0 &&
+          xx < W)
+        u = *reinterpret_cast<const uint4 *>(
+            x + ((((long)n * D + zz) * H + yy) * W + xx) * CIN + c8 * 8);
+      buf[i] = u;
+    }
+  };
+  // transposed into xT[slot][row][ci][x] (16-bit elements)
+  auto putx = [&](int slot, const uint4 (&buf)[XVT]) {
+    u16 *base = reinterpret_cast<u16 *>(xT + slot * PW);
+#pragma unroll
+    for (int i = 0; i < XVT; ++i) {
+      const int v = threadIdx.x + i * NTH;
+      if (v >= XV) continue;
+      const int c8 = v % (CIN / 8), t = v / (CIN / 8);
+      const int xp = t % XR, row = t / XR;
+      const unsigned w4[4] = {buf[i].x, buf[i].y, buf[i].z, buf[i].w};
+      u16 *dst = base + (row * CIN + c8 * 8) * XR + xp;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        dst[(2 * j) * XR] = (u16)(w4[j] & 0xffffu);
+        dst[(2 * j + 1) * XR] = (u16)(w4[j] >> 16);
+      }
+    }
+  };
+  auto fetchd = [&](int zz, uint4 (&buf)[DVT]) {
+#pragma unroll
+    for (int i = 0; i < DVT; ++i) {
+      const int v = threadIdx.x + i * NTH;
+      const int c8 = v % (COUT / 8), t = v / (COUT / 8);
+      const int xp = t % XT, row = t / XT;
+      const int yy = y0 + row, xx = x0 + xp;
+      uint4 u = make_uint4(0, 0, 0, 0);
+      if (v < DV && zz < ze && yy < H && xx < W)
+        u = *reinterpret_cast<const uint4 *>(
+            dz + ((((long)n * D + zz) * H + yy) * W + xx) * COUT + c8 * 8);
+      buf[i] = u;
+    }
+  };
+  auto putd = [&](const uint4 (&buf)[DVT]) {
+#pragma unroll
+    for (int i = 0; i < DVT; ++i) {
+      const int v = threadIdx.x + i * NTH;
+      if (v >= DV) continue;
+      const int c8 = v % (COUT / 8), t = v / (COUT / 8);
+      const unsigned w4[4] = {buf[i].x, buf[i].y, buf[i].z, buf[i].w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        dzT[(c8 * 8 + 2 * j) * (TY * XT) + t] = (u16)(w4[j] & 0xffffu);
+        dzT[(c8 * 8 + 2 * j + 1) * (TY * XT) + t] = (u16)(w4[j] >> 16);
+      }
+    }
+  };
+  if (zb < ze) {
+    uint4 bx[XVT], bd[DVT];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      fetchx(zb - 1 + k, bx);
+      putx(k, bx);
+    }
+    fetchd(zb, bd);
+    putd(bd);
+    fetchx(zb + 2, bx);
+    fetchd(zb + 1, bd);
+    __syncthreads();
+    for (int z = zb; z < ze; ++z) {
+      const int base = z - zb;  // ring slot of plane z - 1 (mod 4)
+      for (int st = 0; st < TY * XT; st += 32) {
+        const int ry = st / XT, rx = st - ry * XT;  // row, x of the step
+        bf16x8 a[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+          const int co = m * 16 + r;
+          a[m] = co < COUT ? *reinterpret_cast<const bf16x8 *>(
+                                 dzT + co * (TY * XT) + st + 8 * g)
+                           : Zero8();
+          if (wv == 0) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) bsum[m] += (float)a[m][j];
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) {
+          if (wv + TY * i >= NTN) break;  // wave-uniform
+          bf16x8 b = Zero8();
+          if (lofs[i] >= 0) {
+            const int e = lofs[i] + ry * CIN * XR + rx + 8 * g;
+            const unsigned *wsrc =
+                xT + ((base + lpl[i]) & 3) * PW + (e >> 1);
+            unsigned w[5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) w[k] = wsrc[k];
+            uint4 o;
+            if (e & 1)
+              o = make_uint4(__builtin_amdgcn_alignbit(w[1], w[0], 16),
+                             __builtin_amdgcn_alignbit(w[2], w[1], 16),
+                             __builtin_amdgcn_alignbit(w[3], w[2], 16),
+                             __builtin_amdgcn_alignbit(w[4], w[3], 16));
+            else
+              o = make_uint4(w[0], w[1], w[2], w[3]);
+            b = __builtin_bit_cast(bf16x8, o);
+          }
+#pragma unroll
+          for (int m = 0; m < MT; ++m)
+            acc[i][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b,
+                                                                acc[i][m], 0,
+                                                                0, 0);
+        }
+      }
+      __syncthreads();  // every read of dzT and of plane z - 1 is done
+      putx((base + 3) & 3, bx);
+      putd(bd);
+      __syncthreads();
+      fetchx(z + 3, bx);
+      fetchd(z + 2, bd);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) {
+    const int col = (wv + TY * i) * 16 + r;
+    if (wv + TY * i >= NTN || col >= K) continue;
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int co = m * 16 + 4 * g + v;
+        if (co < COUT) part[(bid * COUT + co) * K + col] = acc[i][m][v];
+      }
+  }
+  if (wv == 0) {
+    // lanes r (channel 16 m + r) of the four groups g hold disjoint
+    // positions: fold the groups in order
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      float s = bsum[m];
+      s += __shfl_xor(s, 16, 64);
+      s += __shfl_xor(s, 32, 64);
+      const int co = m * 16 + r;
+      if (g == 0 && co < COUT) bpart[bid * COUT + co] = s;
+    }
+  }
+}
+
 // Filter gradient partials: block (b, t) covers positions [b*chunk,
 // (b+1)*chunk) and column tiles 4t .. 4t+3 of k (one per wave) and writes
 // part[b][co][k]; blocks (b, 0) also write bpart[b][co].  A lane decodes its
@@ -685,6 +1054,27 @@ __global__ void PackKernel(const float *__restrict__ w, u16 *__restrict__ wt,
   }
 }
 
+// Reflected copy of whole problems: dst problem q' = dst_first + q holds
+// src problem first + q with the axes whose bit is set in `flips` (1 x,
+// 2 y, 4 z) reversed inside the problem.
+__global__ void ReflectKernel(const float *__restrict__ src,
+                              float *__restrict__ dst, int nx, int ny,
+                              int nzs, long soff, long doff, int flips,
+                              long n) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long m = (long)nx * ny * nzs;
+  const long q = i / m;
+  long r = i - q * m;
+  const int z = (int)(r / ((long)nx * ny));
+  r -= (long)z * nx * ny;
+  const int y = (int)(r / nx), x = (int)(r - (long)y * nx);
+  const int sx = flips & 1 ? nx - 1 - x : x;
+  const int sy = flips & 2 ? ny - 1 - y : y;
+  const int sz = flips & 4 ? nzs - 1 - z : z;
+  dst[doff + i] = src[soff + q * m + ((long)sz * ny + sy) * nx + sx];
+}
+
 __global__ void AdamKernel(float *__restrict__ p, const float *__restrict__ g,
                            float *__restrict__ m, float *__restrict__ v,
                            long n, float lr, float c1, float c2) {
@@ -717,20 +1107,48 @@ void ConvLaunch(const u16 *x, const u16 *wt, const float *b, bool relu,
   Check();
 }
 
-// x tile: 64 positions below 24 input channels, else 32 (measured at
-// batch 8 of 128^3: 8 -> 8 315 us at 64 against 395 at 32; 24 -> 8 1673 us
-// at 32 against 2359 at 64, where the 57 KB tile leaves 2 blocks per CU);
-// 16 for rows of at most 16.
+// z chunks of a z-marching launch: enough blocks for two per compute unit
+// over `tiles` xy tiles of N problems of D planes.
+void ZChunks(long tiles, int N, int D, int *tz, int *zchunk) {
+  const long want = 2L * __PSHipNumCUs();
+  long t = (want + tiles * N - 1) / (tiles * N);
+  if (t < 1) t = 1;
+  if (t > D) t = D;
+  *zchunk = (int)((D + t - 1) / t);
+  *tz = (D + *zchunk - 1) / *zchunk;
+}
+
+template <int CIN, int COUT, int XT, int TY>
+void ConvZLaunch(const u16 *x, const u16 *wt, const float *b, bool relu,
+                 u16 *y, int N, int D, int H, int W) {
+  const int tiles = ((W + XT - 1) / XT) * ((H + TY - 1) / TY);
+  int tz, zchunk;
+  ZChunks(tiles, N, D, &tz, &zchunk);
+  const dim3 grid(tiles, N * tz), block(64 * TY);
+  if (relu)
+    hipLaunchKernelGGL((ConvZKernel<CIN, COUT, true, XT, TY>), grid, block, 0,
+                       Stream(), x, wt, b, y, D, H, W, tz, zchunk);
+  else
+    hipLaunchKernelGGL((ConvZKernel<CIN, COUT, false, XT, TY>), grid, block,
+                       0, Stream(), x, wt, b, y, D, H, W, tz, zchunk);
+  Check();
+}
+
+// z-marching convolution: x tile of 64 positions below 24 input channels,
+// else 32, 16 for rows of at most 16; 8 rows per block below 24 input
+// channels, else 4 (the ring of 4 staged planes then stays within LDS).
 template <int CIN, int COUT>
 void ConvT(const u16 *x, const u16 *wt, const float *b, bool relu, u16 *y,
            int N, int D, int H, int W) {
+  constexpr int TY = CIN < 24 ? 8 : 4;
   if constexpr (CIN < 24) {
-    if (W > 32) return ConvLaunch<CIN, COUT, 64>(x, wt, b, relu, y, N, D, H, W);
+    if (W > 32)
+      return ConvZLaunch<CIN, COUT, 64, TY>(x, wt, b, relu, y, N, D, H, W);
   }
   if (W > 16)
-    ConvLaunch<CIN, COUT, 32>(x, wt, b, relu, y, N, D, H, W);
+    ConvZLaunch<CIN, COUT, 32, TY>(x, wt, b, relu, y, N, D, H, W);
   else
-    ConvLaunch<CIN, COUT, 16>(x, wt, b, relu, y, N, D, H, W);
+    ConvZLaunch<CIN, COUT, 16, TY>(x, wt, b, relu, y, N, D, H, W);
 }
 
 // Positions per filter-gradient block: about P / 512 (a multiple of 32),
@@ -743,24 +1161,70 @@ int WgradChunk(long P) {
   return (int)c;
 }
 
+// Launch shape of the z-marching filter gradient (rows of a multiple of 32
+// positions): x tile 64 when the row allows it and the input has at most
+// 16 channels, else 32; 4 rows per block; z chunks as the convolution.
+// Returns the block count (= partials).
+constexpr int kWgradZRows = 4;
+int WgradZXT(int cin, int W) { return (W % 64 == 0 && cin <= 16) ? 64 : 32; }
+long WgradZBlocks(int cin, int N, int D, int H, int W, int *tz, int *zchunk) {
+  const int xt = WgradZXT(cin, W);
+  const long tiles = (long)((W + xt - 1) / xt) *
+                     ((H + kWgradZRows - 1) / kWgradZRows);
+  ZChunks(tiles, N, D, tz, zchunk);
+  return tiles * N * *tz;
+}
+
+// Partials (blocks) of a filter-gradient launch over N problems of
+// D x H x W positions with cin input channels: every buffer of partials is
+// sized by this.
+long WgradBlocks(int cin, int N, int D, int H, int W) {
+  if (W % 32 == 0) {
+    int tz, zchunk;
+    return WgradZBlocks(cin, N, D, H, W, &tz, &zchunk);
+  }
+  const long P = (long)N * D * H * W;
+  const long chunk = WgradChunk(P);
+  return (P + chunk - 1) / chunk;
+}
+
+// Launches the filter gradient into part / bpart, which hold `capacity`
+// blocks of partials; *nparts receives the blocks written.
 template <int CIN, int COUT>
 void WgradT(const u16 *x, const u16 *dz, float *part, float *bpart, int N,
-            int D, int H, int W, int *nparts) {
+            int D, int H, int W, long capacity, int *nparts) {
   constexpr int NTN = (27 * CIN + 15) / 16;
   const long P = (long)N * D * H * W;
   if (P >= (1L << 31) - kWgradChunk)
     PS_FATAL("filter gradient: %ld positions exceed 32-bit indexing", P);
+  if (WgradBlocks(CIN, N, D, H, W) > capacity)
+    PS_FATAL("filter gradient: %ld partial blocks exceed the %ld allocated",
+             WgradBlocks(CIN, N, D, H, W), capacity);
+  // rows of a multiple of 32 positions: the z-marching kernel
+  if (W % 32 == 0) {
+    int tz, zchunk;
+    const long nb = WgradZBlocks(CIN, N, D, H, W, &tz, &zchunk);
+    const int tiles = (int)(nb / ((long)N * tz));
+    const dim3 grid(tiles, N * tz), block(64 * kWgradZRows);
+    if (WgradZXT(CIN, W) == 64) {
+      if constexpr (CIN <= 16)
+        hipLaunchKernelGGL((WgradZKernel<CIN, COUT, 64, kWgradZRows>), grid,
+                           block, 0, Stream(), x, dz, part, bpart, D, H, W,
+                           tz, zchunk);
+    } else {
+      hipLaunchKernelGGL((WgradZKernel<CIN, COUT, 32, kWgradZRows>), grid,
+                         block, 0, Stream(), x, dz, part, bpart, D, H, W, tz,
+                         zchunk);
+    }
+    Check();
+    *nparts = (int)nb;
+    return;
+  }
   const int chunk = WgradChunk(P);
   const int nb = (int)((P + chunk - 1) / chunk);
-  // rows of a multiple of 32 positions: chunks (multiples of 32) then
-  // split into 32-position steps that never leave a row
-  if (W % 32 == 0)
-    hipLaunchKernelGGL((WgradLdsKernel<CIN, COUT>), dim3(nb), dim3(256), 0,
-                       Stream(), x, dz, part, bpart, (int)P, D, H, W, chunk);
-  else
-    hipLaunchKernelGGL((WgradKernel<CIN, COUT>), dim3(nb, (NTN + 3) / 4),
-                       dim3(256), 0, Stream(), x, dz, part, bpart, (int)P, D,
-                       H, W, chunk);
+  hipLaunchKernelGGL((WgradKernel<CIN, COUT>), dim3(nb, (NTN + 3) / 4),
+                     dim3(256), 0, Stream(), x, dz, part, bpart, (int)P, D,
+                     H, W, chunk);
   Check();
   *nparts = nb;
 }
@@ -794,10 +1258,11 @@ void Conv(int cin, int cout, const u16 *x, const u16 *wt, const float *b,
 }
 
 void Wgrad(int cin, int cout, const u16 *x, const u16 *dz, float *part,
-           float *bpart, int N, int D, int H, int W, int *nparts) {
+           float *bpart, int N, int D, int H, int W, long capacity,
+           int *nparts) {
 #define PS_CASE(I, O) \
   if (cin == I && cout == O) \
-    return WgradT<I, O>(x, dz, part, bpart, N, D, H, W, nparts);
+    return WgradT<I, O>(x, dz, part, bpart, N, D, H, W, capacity, nparts);
   PS_UNET_PAIRS(PS_CASE)
 #undef PS_CASE
   PS_FATAL("no filter-gradient kernel for %d -> %d channels", cin, cout);
@@ -848,7 +1313,7 @@ struct __PSUNet {
   u16 *g3a = nullptr, *g3b = nullptr;                  // level 3
   float *part = nullptr, *bpart = nullptr, *tmp = nullptr;
   double *lpart = nullptr, *lhost = nullptr;
-  size_t part_elems = 0;
+  size_t part_elems = 0, bpart_elems = 0;
 };
 
 namespace {
@@ -950,8 +1415,10 @@ void LayerWgrad(__PSUNet *net, int l, const u16 *in, const u16 *dz, int N,
                 int D, int H, int W) {
   const Layer &L = net->layers[l];
   int np = 0;
-  Wgrad(L.cin, L.cout, in, dz, net->part, net->bpart, N, D, H, W, &np);
   const long nw = (long)L.cout * 27 * L.cin;
+  const long cap = std::min<long>((long)net->part_elems / nw,
+                                  (long)net->bpart_elems / L.cout);
+  Wgrad(L.cin, L.cout, in, dz, net->part, net->bpart, N, D, H, W, cap, &np);
   SumParts(net->part, np, nw, net->tmp, net->grad + L.w_off);
   SumParts(net->bpart, np, (long)L.cout, net->tmp, net->grad + L.b_off);
 }
@@ -1164,15 +1631,25 @@ __PSUNet *PSUNetNew(int nx, int ny, int nzs, int max_batch, float out_scale,
   net->g3b = A16(d.P3 * 32);
   size_t pe = (size_t)kHeadBlocks * 9, be = 0, te = 9;
   for (int l = 0; l < 10; ++l) {
-    const long P = l < 2 || l > 7 ? d.P1 : (l < 4 || l > 5 ? d.P2 : d.P3);
-    const long chunk = WgradChunk(P);
-    const size_t nb = (size_t)((P + chunk - 1) / chunk);
+    const int lv = l < 2 || l > 7 ? 1 : (l < 4 || l > 5 ? 2 : 3);
+    const long P = lv == 1 ? d.P1 : (lv == 2 ? d.P2 : d.P3);
+    const int W = lv == 1 ? d.W1 : (lv == 2 ? d.W2 : d.W3);
+    const int H = lv == 1 ? d.H1 : (lv == 2 ? d.H2 : d.H3);
+    const int D = lv == 1 ? d.D1 : (lv == 2 ? d.D2 : d.D3);
+    (void)P;
+    // the block count depends on the batch: the largest over every batch a
+    // call may use
+    size_t nb = 0;
+    for (int nbat = 1; nbat <= max_batch; ++nbat)
+      nb = std::max<size_t>(
+          nb, (size_t)WgradBlocks(net->layers[l].cin, nbat, D, H, W));
     const size_t nw = (size_t)net->layers[l].cout * 27 * net->layers[l].cin;
     pe = std::max(pe, nb * nw);
     be = std::max(be, nb * net->layers[l].cout);
     te = std::max(te, nw);
   }
   net->part_elems = pe;
+  net->bpart_elems = be;
   net->part = A32((long)pe);
   net->bpart = A32((long)be);
   net->tmp = A32((long)(te * kSumSegments));
@@ -1233,6 +1710,27 @@ void PSUNetPredict(__PSUNet *net, const __PSGrid *rhs, const __PSGrid *bnd,
     hipLaunchKernelGGL(PredictKernel<float>, dim3(Blocks(d.P1)), dim3(256), 0,
                        Stream(), net->out, (const float *)bnd->p,
                        (float *)p->p, off, net->out_scale, d.P1);
+  Check();
+  PS_HIP_CHECK(hipStreamSynchronize(Stream()));
+}
+
+void PSUNetReflect(const __PSGrid *src, int first, int count, __PSGrid *dst,
+                   int dst_first, int nx, int ny, int nzs, int flips) {
+  if (!src || !dst || src->type != PS_FLOAT || dst->type != PS_FLOAT ||
+      src->num_dims != 3 || dst->num_dims != 3 || src->dim[0] != nx ||
+      dst->dim[0] != nx || src->dim[1] != ny || dst->dim[1] != ny ||
+      nzs < 1 || src->dim[2] % nzs || dst->dim[2] % nzs || first < 0 ||
+      dst_first < 0 || count < 0 || flips < 0 || flips > 7 ||
+      (long)(first + count) * nzs > src->dim[2] ||
+      (long)(dst_first + count) * nzs > dst->dim[2] ||
+      (src->p == dst->p && count > 0))
+    PS_FATAL("PSUNetReflect: bad grids or problem range");
+  const long m = (long)nx * ny * nzs, n = m * count;
+  if (n == 0) return;
+  hipLaunchKernelGGL(ReflectKernel, dim3((unsigned)((n + 255) / 256)),
+                     dim3(256), 0, Stream(), (const float *)src->p,
+                     (float *)dst->p, nx, ny, nzs, m * first, m * dst_first,
+                     flips, n);
   Check();
   PS_HIP_CHECK(hipStreamSynchronize(Stream()));
 }
@@ -1322,7 +1820,7 @@ void PSConv3dBackwardFilterHost(const uint16_t *x, const uint16_t *dy, int n,
                                 float *dw, float *db) {
   CheckLayerArgs(n, d, h, wd, cin, cout);
   const long P = (long)n * d * h * wd;
-  const long nb = (P + WgradChunk(P) - 1) / WgradChunk(P);
+  const long nb = WgradBlocks(cin, n, d, h, wd);
   const size_t nw = (size_t)cout * 27 * cin;
   DevBuf ddx(P * cin * 2), ddy(P * cout * 2), part(nb * nw * 4),
       bpart(nb * cout * 4), dw_d(nw * 4), db_d(cout * 4),
@@ -1331,7 +1829,7 @@ void PSConv3dBackwardFilterHost(const uint16_t *x, const uint16_t *dy, int n,
   PS_HIP_CHECK(hipMemcpy(ddy.p, dy, P * cout * 2, hipMemcpyHostToDevice));
   int np = 0;
   Wgrad(cin, cout, (const u16 *)ddx.p, (const u16 *)ddy.p, (float *)part.p,
-        (float *)bpart.p, n, d, h, wd, &np);
+        (float *)bpart.p, n, d, h, wd, nb, &np);
   SumParts((const float *)part.p, np, (long)nw, (float *)tmp.p,
            (float *)dw_d.p);
   SumParts((const float *)bpart.p, np, (long)cout, (float *)tmp.p,

@@ -193,8 +193,9 @@ def unet_md(kt_dir: str, out_md: str, command: str) -> None:
     rows = []
     for f in glob.glob(os.path.join(kt_dir, "*_kernel_stats.csv")):
         for r in csv.DictReader(open(f)):
-            m = re.search(r"ConvKernel<(\d+), (\d+), (true|false)>", r["Name"])
-            w = re.search(r"Wgrad(?:Lds)?Kernel<(\d+), (\d+)>",
+            m = re.search(r"ConvZ?Kernel<(\d+), (\d+), (true|false)",
+                          r["Name"])
+            w = re.search(r"Wgrad(?:Lds|Z)?Kernel<(\d+), (\d+)",
                           r["Name"])
             if m:
                 cin, cout = int(m.group(1)), int(m.group(2))

@@ -44,6 +44,8 @@ def library() -> C.CDLL:
         L.PSUNetTrainStep.argtypes = [C.c_void_p, G, G, G, C.c_int, C.c_int,
                                       C.c_float]
         L.PSUNetTrainStep.restype = C.c_double
+        L.PSUNetReflect.argtypes = [G, C.c_int, C.c_int, G, C.c_int,
+                                    C.c_int, C.c_int, C.c_int, C.c_int]
         L.PSConv3dForwardHost.argtypes = [U16P, U16P, F32P, C.c_int] + \
             [C.c_int] * 6 + [U16P]
         L.PSConv3dBackwardDataHost.argtypes = [U16P, U16P] + [C.c_int] * 6 + \
@@ -122,6 +124,14 @@ class UNet:
                 print(f"  train step {s}: loss {hist[-1]:.4e} "
                       f"({time.perf_counter() - t0:.1f} s)", flush=True)
         return hist
+
+
+def reflect(src, first, count, dst, dst_first, shape, flips) -> None:
+    """Problems [first, first + count) of src (problems of shape = (nx, ny,
+    nzs) stacked along z) into problems [dst_first, ...) of dst with the
+    axes in `flips` (bit 0 x, 1 y, 2 z) reversed."""
+    library().PSUNetReflect(src.handle, first, count, dst.handle, dst_first,
+                            *shape, flips)
 
 
 def conv_forward_host(x, w, b, relu, n, d, h, wd, cin, cout):

@@ -95,7 +95,8 @@ def test_prediction_over_the_batch(ps, batch):
     net = UNet(N, N, N, 8, 2.0, seed=11)
     pg = ps.PSGridNew(SHAPE)
     try:
-        net.predict(grids[11], grids[10], 0, NB, pg)
+        for f0 in range(0, NB, 8):  # the network runs batches of 8
+            net.predict(grids[11], grids[10], f0, 8, pg)
         got = ps.PSGridCopyout(pg)
         assert np.all(np.isfinite(got))
         assert np.all(got[coefs[10] == 0] == 0)
