@@ -236,7 +236,73 @@ def unet_md(kt_dir: str, out_md: str, command: str) -> None:
     print("\n".join(lines))
 
 
+def counters_md(dirs, out_md: str, command: str) -> None:
+    """Per-launch averages of the instruction / stall counters of
+    `rocprofv3 --pmc` runs (one directory per counter group) for the
+    7-point run kernels, with derived rates: busy and wait fractions of the
+    wave cycles, instructions per wave, LDS bank conflicts per LDS
+    instruction."""
+    frags = [("two-sweep 7-pt (clamped)", "__PSStencilRun2_diffusion7_kernel<"),
+             ("one-sweep 7-pt (clamped)", "__PSStencilRun_diffusion7_kernel<"),
+             ("two-sweep 7-pt (periodic)",
+              "__PSStencilRun2_diffusion7_periodic_kernel<"),
+             ("512 MiB copy", "copyBuffer")]
+    vals = defaultdict(dict)
+    for d in dirs:
+        for f in glob.glob(os.path.join(d, "**", "*_counter_collection.csv"),
+                           recursive=True):
+            acc = defaultdict(list)
+            for r in csv.DictReader(open(f)):
+                for label, frag in frags:
+                    if _matches(r["Kernel_Name"], frag) or (
+                            frag == "copyBuffer" and frag in r["Kernel_Name"]):
+                        acc[(label, r["Counter_Name"])].append(
+                            float(r["Counter_Value"]))
+            for (label, c), v in acc.items():
+                vals[label][c] = sum(v) / len(v)
+    names = sorted({c for v in vals.values() for c in v})
+    lines = ["# Instruction and stall counters of the 7-point run kernels "
+             "(MI355X)", "", "```", command, "```", "",
+             "Per-launch averages (rocprofv3 sums each counter over the "
+             "device for a dispatch).", "",
+             "| counter | " + " | ".join(l for l, _ in frags if l in vals) + " |",
+             "|---|" + "---|" * len([1 for l, _ in frags if l in vals])]
+    for c in names:
+        lines.append(f"| {c} | " + " | ".join(
+            f"{vals[l].get(c, float('nan')):.4g}" for l, _ in frags
+            if l in vals) + " |")
+    lines += ["", "Derived:", ""]
+    for l, _ in frags:
+        v = vals.get(l)
+        if not v:
+            continue
+        parts = []
+        wc = v.get("SQ_WAVE_CYCLES")
+        waves = v.get("SQ_WAVES")
+        if wc:
+            for k in ("SQ_BUSY_CYCLES", "SQ_WAIT_ANY", "SQ_WAIT_INST_ANY",
+                      "SQ_ACTIVE_INST_ANY", "SQ_ACTIVE_INST_VALU",
+                      "SQ_ACTIVE_INST_LDS", "SQ_WAIT_INST_LDS"):
+                if k in v:
+                    parts.append(f"{k} / SQ_WAVE_CYCLES = {v[k] / wc:.3f}")
+        if waves:
+            for k in ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS",
+                      "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR"):
+                if k in v:
+                    parts.append(f"{k} per wave = {v[k] / waves:.1f}")
+        if "SQ_LDS_BANK_CONFLICT" in v and "SQ_ACTIVE_INST_LDS" in v:
+            parts.append("SQ_LDS_BANK_CONFLICT / SQ_ACTIVE_INST_LDS = "
+                         f"{v['SQ_LDS_BANK_CONFLICT'] / v['SQ_ACTIVE_INST_LDS']:.3f}")
+        lines.append(f"* {l}: " + "; ".join(parts))
+    lines.append("")
+    open(out_md, "w").write("\n".join(lines))
+    print("\n".join(lines))
+
+
 if __name__ == "__main__":
+    if len(sys.argv) >= 5 and sys.argv[1] == "--counters":
+        counters_md(sys.argv[2:-2], sys.argv[-2], sys.argv[-1])
+        sys.exit(0)
     if len(sys.argv) == 5 and sys.argv[1] == "--unet":
         unet_md(*sys.argv[2:])
     elif len(sys.argv) == 5 and sys.argv[1] == "--ktrace":
