@@ -362,6 +362,11 @@ struct Stage {
     for (int i = XL; i < VX + XR; ++i)
       seg[i - XL] = (i >= 0 && i < VX) ? mid[i] : p[i];
   }
+  // Writes the lane's VX values of row `row` of a staged plane.
+  __device__ __forceinline__ static void WriteRow(T *slot, int row, int lane,
+                                                  const T (&v)[VX]) {
+    StoreVec(slot + row * kRow + kPad + lane * VX, v);
+  }
 
  private:
   __device__ __forceinline__ static void StoreVec(T *dst, const T (&v)[VX]) {

@@ -242,8 +242,11 @@ __device__ __forceinline__ M UpdateM(const float (&c)[kNumF], float pc,
 // of wave 0) and E''(s-2) on the tile; one barrier per step.  Points with
 // bnd = 0 (every problem's faces, and points outside the grid, whose
 // fields load as 0) keep their value, so the E' values the pass computes
-// beyond the grid feed only points that ignore them.  The fields are read
-// for E' one step ahead and again, two planes later, for E''.
+// beyond the grid feed only points that ignore them.  With 16-bit storage
+// and two points per lane the fields of plane s are read once, one step
+// ahead of E'(s), and stay in a register ring of 4 planes until E''(s) two
+// steps later (the rows of E'' are rows of E'); otherwise (the ring would
+// not fit the register budget) they are read again for E''.
 template <typename M, int VX, int BY>
 __global__ __launch_bounds__(kWave *BY) void TwoSweepKernel(
     const M *__restrict__ pin, M *__restrict__ pout, Fields<M> f, int nx,
@@ -288,7 +291,9 @@ __global__ __launch_bounds__(kWave *BY) void TwoSweepKernel(
   }
 #pragma unroll
   for (int k = 0; k < kPD; ++k) st[k].Load(aw, pin, dim, ra, zb + 1 + k);
-  M f1[kNumF][2][VX], f2[kNumF][VX];
+  constexpr bool kRing = sizeof(M) == 2 && VX == 2;
+  // kRing: plane s in slot (s - zb + 1) & 3; else slots 0/1 alternate
+  M f1[kNumF][kRing ? 4 : 2][VX], f2[kNumF][kRing ? 1 : VX];
   // fields of plane z for E': the lane's VX points on rows y0-1 .. y0+TY,
   // its halo point in wave 0
   auto load1 = [&](int z, int slot) {
@@ -310,14 +315,18 @@ __global__ __launch_bounds__(kWave *BY) void TwoSweepKernel(
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int s = sc + k;
-      const int cur = k & 1, nxt = cur ^ 1;
+      const int cur = kRing ? k : k & 1;
+      const int nxt = kRing ? (k + 1) & 3 : cur ^ 1;
       if (s > ze + 1) break;  // uniform over the block
       load1(s + 1, nxt);
-      const int o2 = out_row && s - 2 >= zb ? xs + yrow * nx + (s - 2) * nxy
-                                            : physis_hip::kDrop;
+      if constexpr (!kRing) {
+        const int o2 = out_row && s - 2 >= zb
+                           ? xs + yrow * nx + (s - 2) * nxy
+                           : physis_hip::kDrop;
 #pragma unroll
-      for (int g = 0; g < kNumF; ++g)
-        fw[g].template Load<VX, physis_hip::kStreamLoad>(o2, f2[g]);
+        for (int g = 0; g < kNumF; ++g)
+          fw[g].template Load<VX, physis_hip::kStreamLoad>(o2, f2[g]);
+      }
       if (mid_row && s <= ze) {
         Rows19<float, VX> r;
         r.template Read<SA>(a_lds, k, wave, lane);
@@ -353,7 +362,9 @@ __global__ __launch_bounds__(kWave *BY) void TwoSweepKernel(
         for (int i = 0; i < VX; ++i) {
           float c[kNumF];
 #pragma unroll
-          for (int g = 0; g < kNumF; ++g) c[g] = Convert<float, M>(f2[g][i]);
+          for (int g = 0; g < kNumF; ++g)
+            c[g] = Convert<float, M>(kRing ? f1[g][(k + 2) & 3][i]
+                                           : f2[g][kRing ? 0 : i]);
           out[i] = UpdateM<M>(c, r.Centre(i), r.S0(c, i), omega);
           if (xs + i < nx) mask |= 1u << i;
         }

@@ -1652,11 +1652,11 @@ std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
        "// computes u' = " << n << "(src) on the tile and a one-point halo and\n"
        "// u'' = " << n << "(u') two planes behind; dst <- u'', mid <- u' when\n"
        "// mid is not NULL.\n";
-  o << "template <int BY>\n__global__ __launch_bounds__(64 * BY) void __PSStencilRun2_"
+  o << "template <int VX, int BY>\n__global__ __launch_bounds__(64 * BY) void __PSStencilRun2_"
     << n << "(\n    __PSDomain dom, " << V << " src, " << V << " dst, " << T
     << " *__restrict__ mid" << (sp.empty() ? "" : ", " + Join(sp, ", "))
     << ", physis_hip::RunShape sh) {\n";
-  o << "  constexpr int VX = 4, PD = 2, TX = 64 * VX, TY = BY - 4;\n"
+  o << "  constexpr int PD = 2, TX = 64 * VX, TY = BY - 4;\n"
     << "  typedef physis_hip::Stage<" << T << ", VX, BY, 2, 2, 4, " << per
     << "> SA;\n"
     << "  typedef physis_hip::Stage<" << T << ", VX, BY, 1, 1, 4, false> SB;\n"
@@ -1717,11 +1717,8 @@ std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
       << "          uo[i] = ctx." << O << "_v;\n        }\n";
     o << SplitInterior(k, "yrow", "s", b.str(), "        ");
   }
-  o << "        " << T << " *brow = b_lds + physis_hip::RingSlot<4>(bbase, 3) * "
-       "SB::kPlane + wave * SB::kRow + SB::kPad;\n"
-       "        *reinterpret_cast<physis_hip::NativeVec<" << T
-    << ", 4>::V *>(brow + lane * VX) =\n            physis_hip::NativeVec<" << T
-    << ", 4>::V{uo[0], uo[1], uo[2], uo[3]};\n";
+  o << "        SB::WriteRow(b_lds + physis_hip::RingSlot<4>(bbase, 3) * "
+       "SB::kPlane, wave, lane, uo);\n";
   o << "        if (mid && out_row && s >= zb && s < ze) {\n"
        "          unsigned m = 0;\n"
        "#pragma unroll\n          for (int i = 0; i < VX; ++i)\n"
@@ -1766,26 +1763,28 @@ std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
        "      __syncthreads();\n"
        "      st[k % PD].Load(aw, src.p, src.dim, ra, s + 2 + PD);\n"
        "    }\n  }\n}\n\n";
-  // launcher.  Block height: 12 waves (8 output rows, one block per CU by
-  // LDS) ran the 512^3 clamped kernel 4% faster than 16 waves (12 output
-  // rows) and as fast as 8 waves in two blocks per CU, measured on one box.
-  const int by = 12;
-  const std::string BY = std::to_string(by), TYS = std::to_string(by - 4);
+  // launcher.  Launch shape: 4 points per lane in 16-wave blocks (12
+  // output rows, one block per CU by LDS); measured on the 512^3 clamped
+  // kernel in one run: 258 us per launch, against 294 in 12-wave blocks
+  // and 301 / 373 / 431 us at 2 points per lane in 12 / 8 / 16-wave blocks.
+  const int vx = 4, by = 16;
+  o << "static constexpr int __PS2VX_" << n << " = " << vx << ", __PS2BY_"
+    << n << " = " << by << ";\n";
   std::vector<std::string> largs = {"s.dom", "src", "dst", "mid"};
   for (auto &a : sa) largs.push_back("s." + a);
   largs.push_back("sh");
   o << "static void __PSLaunch2_" << n << "(const struct __PSStencil_" << n
     << " &s, const " << V << " &src, const " << V << " &dst, " << T
     << " *mid, hipStream_t stream) {\n"
+    << "  constexpr int VX = __PS2VX_" << n << ", BY = __PS2BY_" << n << ";\n"
     << "  static int per_cu = physis_hip::BlocksPerCU(\n"
-       "      (const void *)__PSStencilRun2_" << n << "<" << BY << ">, 64 * "
-    << BY << ");\n"
-    << "  const physis_hip::RunShape sh = physis_hip::PlanRun(s.dom, 256, "
-    << TYS << ", per_cu,\n"
+       "      (const void *)__PSStencilRun2_" << n << "<VX, BY>, 64 * BY);\n"
+    << "  const physis_hip::RunShape sh = physis_hip::PlanRun(s.dom, 64 * VX, "
+       "BY - 4, per_cu,\n"
        "      physis_hip::WindowChunkLimit({s." << I << "}, 3));\n"
     << "  if (sh.blocks == 0) return;\n"
-    << "  hipLaunchKernelGGL((__PSStencilRun2_" << n << "<" << BY
-    << ">), dim3(sh.blocks), dim3(64 * " << BY << "), 0, stream, "
+    << "  hipLaunchKernelGGL((__PSStencilRun2_" << n << "<VX, BY>), "
+       "dim3(sh.blocks), dim3(64 * BY), 0, stream, "
     << Join(largs, ", ") << ");\n}\n\n";
   // run-time check
   o << "// The pair (s0, s1) is " << n << "(a -> b), " << n
@@ -2162,11 +2161,12 @@ std::string Translator::EmitFusedDomainKernel(const KernelInfo &k) {
        "      st[0].Store(a_lds + k * SA::kPlane, wave, lane);\n    }\n"
        "#pragma unroll\n    for (int k = 0; k < PD; ++k)\n"
        "      st[k].Load(aw, src.p, src.dim, ra, zb + 1 + k);\n";
-  // centre values of u' (one step ahead) and of u''
+  // centre values of the direct grids: plane s is loaded one step ahead of
+  // u'(s) and kept for u''(s) two steps later (u'' rows are u' rows), in
+  // a ring of 4 planes indexed by the unrolled step
   for (auto *g : direct)
-    o << "    " << g->elem << " " << g->name << "_c1[2][VX], " << g->name
-      << "_c2[VX];\n";
-  o << "    " << T << " bg_c1[2][VX];\n";
+    o << "    " << g->elem << " " << g->name << "_c1[4][VX];\n";
+  o << "    " << T << " bg_c1[4][VX];\n";
   o << "    // loads of plane z for u': the lane's VX points on mid rows, its halo\n"
        "    // point in wave 0; emitted-grid values only where a point lies\n"
        "    // outside the domain\n"
@@ -2199,13 +2199,10 @@ std::string Translator::EmitFusedDomainKernel(const KernelInfo &k) {
        "      for (int k = 0; k < 4; ++k) {\n"
        "        const int s = sc + k;\n"
        "        const int abase = k, bbase = k;\n"
-       "        const int cur = k & 1, nxt = cur ^ 1;\n"
-       "        if (s > ze + 1) break;  // uniform over the block\n"
-       "        load1(s + 1, nxt);\n"
-       "        const int o2 = out_row && s - 2 >= zb ? xs + yrow * nx + (s - 2) * nxy : physis_hip::kDrop;\n";
-  for (auto *g : direct)
-    o << "        " << g->name << "_w.template Load<VX, physis_hip::kStreamLoad>(o2, "
-      << g->name << "_c2);\n";
+       "        const int cur = k, nxt = (k + 1) & 3;\n";
+  if (!direct.empty()) o << "        const int old = (k + 2) & 3;\n";
+  o << "        if (s > ze + 1) break;  // uniform over the block\n"
+       "        load1(s + 1, nxt);\n";
   // u' on the lane's points
   o << "        if (mid_row && s <= ze) {\n";
   o << read_rows("a", "a_lds", "SA", "abase", "          ");
@@ -2255,7 +2252,7 @@ std::string Translator::EmitFusedDomainKernel(const KernelInfo &k) {
        "#pragma unroll\n          for (int i = 0; i < VX; ++i) {\n";
   o << fill("b", "            ");
   for (auto *g : direct)
-    o << "            ctx." << g->name << "_c = " << g->name << "_c2[i];\n";
+    o << "            ctx." << g->name << "_c = " << g->name << "_c1[old][i];\n";
   o << "            ctx." << O << "_e = false;\n"
     << "            const bool in = xs + i >= dom.local_min[0] && xs + i < dom.local_max[0];\n";
   if (red)
@@ -2284,14 +2281,13 @@ std::string Translator::EmitFusedDomainKernel(const KernelInfo &k) {
        "      if (red2) red2[blockIdx.x] = r2;\n    }\n"
        "  } else {\n    (void)acc1; (void)acc2; (void)red1; (void)red2;\n  }\n}\n\n";
   // launcher, block count and run-time check.  Every centre-only grid
-  // holds 2 * VX values of u' in flight and VX of u'' per lane.  Without
-  // such grids the kernel is as light as the whole-grid one (16-wave
-  // blocks); with many (the 19-point Himeno kernel's 12), two points per
-  // lane and 12-wave blocks stay within the register file at three waves
-  // per SIMD (measured at 512x256x256: 107 Gpts/s with the per-sweep
-  // residual, against 107 for VX = 4 in 8-wave blocks and 91 for single
-  // sweeps).
-  const int kVX = direct.size() > 4 ? 2 : 4;
+  // holds 4 * VX values per lane (its ring of 4 planes).  Without such
+  // grids the kernel is as light as the whole-grid one (16-wave blocks);
+  // with many (the 19-point Himeno kernel's 12), one point per lane keeps
+  // the rings within the register file at four waves per SIMD (measured at
+  // 512x256x256 with the per-sweep residual: 132 Gpts/s in 12-wave blocks,
+  // 125 in 16-wave blocks, 101 for two points per lane in 8-wave blocks).
+  const int kVX = direct.size() > 4 ? 1 : 4;
   const int kBY = direct.empty() ? 16 : (direct.size() > 4 ? 12 : 8);
   o << "static constexpr int __PS2VX_" << n << " = " << kVX << ", __PS2BY_" << n
     << " = " << kBY << ";\n";
