@@ -82,7 +82,7 @@ def oracle_lib():
     L.oracle_himeno_run.argtypes = [FP, FP, C.c_void_p] + [C.c_int] * 4 + \
         [F, C.c_void_p]
     L.oracle_himeno_run_omp.argtypes = [FP, FP, C.c_void_p] + \
-        [C.c_int] * 4 + [F, C.c_int]
+        [C.c_int] * 4 + [F, C.c_int, C.c_void_p]
     L.oracle_himeno_run_omp.restype = C.c_int
     return L
 
@@ -161,7 +161,8 @@ def cpu_baselines(a: np.ndarray, coef) -> dict:
     t_ref = time.perf_counter() - t0
     p0, p1 = p.copy(), p.copy()
     t0 = time.perf_counter()
-    th = L.oracle_himeno_run_omp(p0, p1, arr, nx, ny, nz, s_omp, 0.8, div)
+    th = L.oracle_himeno_run_omp(p0, p1, arr, nx, ny, nz, s_omp, 0.8, div,
+                                  None)
     t_omp = time.perf_counter() - t0
     # the reference's Himeno throughput counts 56 B/pt of its 14 grids
     out["19pt_himeno"] = {

@@ -5,6 +5,7 @@
 
 #include <math.h>
 #include <stddef.h>
+#include <stdlib.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -207,9 +208,10 @@ void oracle_himeno_run(float *p0, float *p1, const float *const *coef, int nx,
 
 int oracle_himeno_run_omp(float *p0, float *p1, const float *const *coef,
                           int nx, int ny, int nz, int nn, float omega,
-                          int div) {
+                          int div, double *gosa) {
   int threads = 1;
   if (div < 1) div = 1;
+  double *part = gosa ? (double *)calloc((size_t)div, sizeof(double)) : NULL;
   for (int i = 0; i < nn; ++i) {
     const float *in = (i % 2 == 0) ? p0 : p1;
     float *out = (i % 2 == 0) ? p1 : p0;
@@ -217,12 +219,20 @@ int oracle_himeno_run_omp(float *p0, float *p1, const float *const *coef,
     for (int u = 0; u < div; ++u) {
       int z0, z1;
       unit_range(1, nz - 1, div, u, &z0, &z1);
-      (void)himeno_zrange(in, out, coef, nx, ny, z0, z1, omega);
+      double g = himeno_zrange(in, out, coef, nx, ny, z0, z1, omega);
+      if (part) part[u] = g;
 #ifdef _OPENMP
       if (u == 0) threads = omp_get_num_threads();
 #endif
     }
+    if (part) {
+      /* unit partials (each in z-y-x order) folded in unit order */
+      double g = 0.0;
+      for (int u = 0; u < div; ++u) g += part[u];
+      gosa[i] = g;
+    }
   }
+  free(part);
   return threads;
 }
 

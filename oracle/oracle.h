@@ -89,9 +89,13 @@ void oracle_himeno_sweep(const float *p0, float *p1, const float *const *coef,
  * receives sweep i's FP64 residual sum if gosa is non-NULL. */
 void oracle_himeno_run(float *p0, float *p1, const float *const *coef, int nx,
                        int ny, int nz, int nn, float omega, double *gosa);
+/* The same sweeps with the z range split into div units run by OpenMP
+ * threads (the shape of the reference's MPI-OpenMP target); gosa[i], if
+ * non-NULL, is the sum of the units' FP64 partials in unit order.  Returns
+ * the number of threads used. */
 int oracle_himeno_run_omp(float *p0, float *p1, const float *const *coef,
                           int nx, int ny, int nz, int nn, float omega,
-                          int div);
+                          int div, double *gosa);
 
 /* ---- system-test kernels (tests/system_tests/test_cases), N^3 float/int
  * grids, each one map application in -> out over the test's domain. */

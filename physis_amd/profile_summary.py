@@ -290,7 +290,8 @@ def counters_md(dirs, out_md: str, command: str) -> None:
                       "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR"):
                 if k in v:
                     parts.append(f"{k} per wave = {v[k] / waves:.1f}")
-        if "SQ_LDS_BANK_CONFLICT" in v and "SQ_ACTIVE_INST_LDS" in v:
+        if v.get("SQ_LDS_BANK_CONFLICT") is not None and \
+                v.get("SQ_ACTIVE_INST_LDS"):
             parts.append("SQ_LDS_BANK_CONFLICT / SQ_ACTIVE_INST_LDS = "
                          f"{v['SQ_LDS_BANK_CONFLICT'] / v['SQ_ACTIVE_INST_LDS']:.3f}")
         lines.append(f"* {l}: " + "; ".join(parts))

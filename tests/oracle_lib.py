@@ -62,7 +62,7 @@ def lib() -> C.CDLL:
                                         C.c_int, C.c_int, F, C.c_void_p]
         L.oracle_himeno_run_omp.argtypes = [FP, FP, C.c_void_p, C.c_int,
                                             C.c_int, C.c_int, C.c_int, F,
-                                            C.c_int]
+                                            C.c_int, C.c_void_p]
         L.oracle_himeno_run_omp.restype = C.c_int
         for n in ("oracle_test_7pt", "oracle_test_7pt_periodic",
                   "oracle_test_7pt_neumann", "oracle_test_asymmetric",

@@ -287,11 +287,15 @@ def test_himeno_omp_shape_is_bit_identical():
     coefs, p = himeno_inputs(nx, ny, nz, 11)
     a0, a1 = p.copy(), p.copy()
     b0, b1 = p.copy(), p.copy()
+    ga, gb = np.zeros(4), np.zeros(4)
     lib().oracle_himeno_run(a0, a1, coef_array(coefs), nx, ny, nz, 4,
-                            np.float32(0.8), None)
+                            np.float32(0.8), ga.ctypes.data)
     lib().oracle_himeno_run_omp(b0, b1, coef_array(coefs), nx, ny, nz, 4,
-                                np.float32(0.8), 4)
+                                np.float32(0.8), 4, gb.ctypes.data)
     assert np.array_equal(a0, b0)
+    # the same FP64 terms summed in another grouping
+    np.testing.assert_allclose(gb, ga, rtol=1e-12)
+    assert np.all(ga > 0)
 
 
 # ----------------------------------------------------------- reductions
