@@ -56,12 +56,20 @@ void PSUNetSetParams(__PSUNet *net, const float *host);
 void PSUNetPredict(__PSUNet *net, const __PSGrid *rhs, const __PSGrid *bnd,
                    int first, int count, __PSGrid *p);
 
-/* Loss and gradient without an update: mean over points of
- * (out - target / out_scale)^2 for problems [first, first + count);
- * grad_host (NumParams floats) may be NULL. */
+/* Loss and gradient without an update, for problems [first, first + count)
+ * with d = out - target / out_scale:
+ *   mean over points of d^2
+ *   + w4  * mean over 4x4x4 cells of (the cell's mean of d)^2
+ *   + w16 * mean over 16x16x16 cells of (the cell's mean of d)^2
+ * (w4 = w16 = 0 unless set by PSUNetSetLossWeights: the mean square error).
+ * The two cell terms weight the smooth part of the error, which Jacobi
+ * sweeps remove slowest.  grad_host (NumParams floats) may be NULL. */
 double PSUNetLossGrad(__PSUNet *net, const __PSGrid *rhs, const __PSGrid *bnd,
                       const __PSGrid *target, int first, int count,
                       float *grad_host);
+
+/* Weights w4, w16 >= 0 of the smooth-error terms of the loss. */
+void PSUNetSetLossWeights(__PSUNet *net, float w4, float w16);
 
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on that loss; returns the
  * loss before the step. */

@@ -85,14 +85,21 @@ def forward(params, rhs, bnd, shape, nbatch, out_scale, rounding=True,
 
 
 def loss_grad(params, rhs, bnd, target, shape, nbatch, out_scale,
-              rounding=True, dtype=torch.float32):
-    """(loss, grad[numpy]) of mean((out - target / out_scale)^2)."""
+              rounding=True, dtype=torch.float32, w4=0.0, w16=0.0):
+    """(loss, grad[numpy]) of mean(d^2) + w4 mean(pool4(d)^2) +
+    w16 mean(pool16(d)^2), d = out - target / out_scale, poolK the mean
+    over K x K x K cells (the header's PSUNetLossGrad)."""
     out, p = forward(params, rhs, bnd, shape, nbatch, out_scale, rounding,
                      dtype, requires_grad=True)
     nx, ny, nzs = shape
     t = torch.as_tensor(np.asarray(target), dtype=dtype).reshape(
         nbatch, nzs, ny, nx) / out_scale
-    loss = ((out - t) ** 2).mean()
+    d = out - t
+    loss = (d ** 2).mean()
+    if w4 or w16:
+        d5 = d.reshape(nbatch, 1, nzs, ny, nx)
+        loss = loss + w4 * (F.avg_pool3d(d5, POOL) ** 2).mean() + \
+            w16 * (F.avg_pool3d(d5, POOL * POOL) ** 2).mean()
     loss.backward()
     return float(loss.detach()), p.grad.numpy().copy()
 

@@ -43,6 +43,10 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kWgradChunk = 16384;  // max positions per filter-gradient block
 constexpr int kPool = 4;           // pooling / upsampling factor per axis
+// Weight fragments (K steps x 16-channel tiles) that the z-marching
+// convolution keeps in registers; more are staged in LDS.  21 covers the
+// 24 -> 8 layer (6% faster than staging its weights).
+constexpr int kRegWeightSteps = 21;
 
 __device__ __forceinline__ float Bf2F(u16 h) {
   return __uint_as_float((unsigned)h << 16);
@@ -179,7 +183,7 @@ __global__ __launch_bounds__(64 * TY) void ConvZKernel(
     int tz, int zchunk) {
   constexpr int K = 27 * CIN, KB = (K + 31) / 32, KP = KB * 32;
   constexpr int MT = (COUT + 15) / 16;
-  constexpr bool kRegW = KB * MT <= 14;
+  constexpr bool kRegW = KB * MT <= kRegWeightSteps;
   constexpr int XS = XT + 2, RS = TY + 2, NV = CIN / 8;
   constexpr int PLANE = RS * XS * CIN;  // u16 elements of a staged plane
   constexpr int SV = RS * XS * NV;      // 16-byte vectors of a staged plane
@@ -324,7 +328,8 @@ __global__ __launch_bounds__(64 * TY) void ConvZKernel(
 // lane's 8 positions of one channel, staged transposed dzT[co][y][x]), B =
 // the shifted input (8 consecutive x of one channel, staged transposed
 // xT[plane][row][ci][x] in a ring of 4 planes, read as 32-bit words with a
-// funnel shift for odd offsets).  Wave w takes column tiles w, w + TY, ...
+// funnel shift for odd offsets; one unaligned 16-byte LDS read instead
+// measured 1.8-2.4x slower).  Wave w takes column tiles w, w + TY, ...
 // against the A operands of every 32-position step of the plane; wave 0
 // also sums dz per channel for the bias.  The planes of the input are
 // staged once per tile.
@@ -980,18 +985,83 @@ __global__ void HeadKernel(const u16 *__restrict__ in,
   out[p] = s + w[8];
 }
 
-// Loss partials and dout = 2 (out - t) / P with t = target * inv_scale.
+// Smooth-error terms of the loss.  The error d = out - target * inv_scale
+// averaged over the 4x4x4 cells of the pooling grid: one wave per cell
+// (lane l at offset (l & 3, (l >> 2) & 3, l >> 4)), summed across the wave
+// in a fixed order.  e4 has the level-2 layout ((n D2 + Z) H2 + Y) W2 + X.
+__global__ __launch_bounds__(256) void SmoothCellKernel(
+    const float *__restrict__ out, const float *__restrict__ target,
+    long toff, float inv_scale, float *__restrict__ e4, int D1, int H1,
+    int W1, long P2) {
+  const long c = (blockIdx.x * (long)blockDim.x + threadIdx.x) >> 6;
+  const int l = threadIdx.x & 63;
+  if (c >= P2) return;  // uniform over the wave
+  const int W2 = W1 / kPool, H2 = H1 / kPool, D2 = D1 / kPool;
+  const int X = (int)(c % W2), Y = (int)((c / W2) % H2);
+  const long nz = c / ((long)W2 * H2);  // n D2 + Z
+  const long n = nz / D2, Z = nz - n * D2;
+  const long p = ((n * D1 + Z * kPool + (l >> 4)) * H1 + Y * kPool +
+                  ((l >> 2) & 3)) * W1 + X * kPool + (l & 3);
+  float v = out[p] - target[toff + p] * inv_scale;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  if (l == 0) e4[c] = v * (1.0f / 64.0f);
+}
+
+// e16: the means of e4 over 4x4x4 cells of the level-2 grid (the error
+// averaged over 16x16x16 cells), one thread per cell, summed z-y-x.
+__global__ void SmoothCoarseKernel(const float *__restrict__ e4,
+                                   float *__restrict__ e16, int D2, int H2,
+                                   int W2, long P3) {
+  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (c >= P3) return;
+  const int W3 = W2 / kPool, H3 = H2 / kPool, D3 = D2 / kPool;
+  const int X = (int)(c % W3), Y = (int)((c / W3) % H3);
+  const long nz = c / ((long)W3 * H3);
+  const long n = nz / D3, Z = nz - n * D3;
+  float s = 0.0f;
+  for (int dz = 0; dz < kPool; ++dz)
+    for (int dy = 0; dy < kPool; ++dy)
+      for (int dx = 0; dx < kPool; ++dx)
+        s += e4[((n * D2 + Z * kPool + dz) * H2 + Y * kPool + dy) * W2 +
+                X * kPool + dx];
+  e16[c] = s * (1.0f / 64.0f);
+}
+
+// Loss partials and its gradient dout.  With d = out - target * inv_scale,
+// e4 / e16 the means of d over the point's 4^3 / 16^3 cell:
+//   loss = (1/P) sum_p (d^2 + w4 e4^2 + w16 e16^2)
+//        = mean d^2 + w4 mean over 4^3 cells e4^2 + w16 mean e16^2,
+//   dout = 2 (d + w4 e4 + w16 e16) / P.
+// SMOOTH false: w4 = w16 = 0 (e4, e16 unused).
+template <bool SMOOTH>
 __global__ __launch_bounds__(256) void LossKernel(
     const float *__restrict__ out, const float *__restrict__ target,
-    long toff, float inv_scale, float *__restrict__ dout,
-    double *__restrict__ part, long P) {
+    long toff, float inv_scale, const float *__restrict__ e4,
+    const float *__restrict__ e16, float w4, float w16, int D1, int H1,
+    int W1, float *__restrict__ dout, double *__restrict__ part, long P) {
   __shared__ double red[4];
   double s = 0.0;
   for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < P;
        p += (long)gridDim.x * blockDim.x) {
     const float d = out[p] - target[toff + p] * inv_scale;
-    dout[p] = 2.0f * d / (float)P;
-    s += (double)d * d;
+    if constexpr (SMOOTH) {
+      const int x = (int)(p % W1), y = (int)((p / W1) % H1);
+      const long nz = p / ((long)W1 * H1);
+      const long n = nz / D1, z = nz - n * D1;
+      const int W2 = W1 / kPool, H2 = H1 / kPool, D2 = D1 / kPool;
+      const long c2 = ((n * D2 + z / kPool) * H2 + y / kPool) * W2 +
+                      x / kPool;
+      const int W3 = W2 / kPool, H3 = H2 / kPool, D3 = D2 / kPool;
+      const long c3 = ((n * D3 + z / (kPool * kPool)) * H3 +
+                       y / (kPool * kPool)) * W3 + x / (kPool * kPool);
+      const float a = e4[c2], b = e16[c3];
+      dout[p] = 2.0f * (d + w4 * a + w16 * b) / (float)P;
+      s += (double)d * d + (double)w4 * a * a + (double)w16 * b * b;
+    } else {
+      dout[p] = 2.0f * d / (float)P;
+      s += (double)d * d;
+    }
   }
   s = physis_hip::BlockReduce<double, PS_SUM, 256>(s, red);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -1107,10 +1177,17 @@ void ConvLaunch(const u16 *x, const u16 *wt, const float *b, bool relu,
   Check();
 }
 
-// z chunks of a z-marching launch: enough blocks for two per compute unit
-// over `tiles` xy tiles of N problems of D planes.
-void ZChunks(long tiles, int N, int D, int *tz, int *zchunk) {
-  const long want = 2L * __PSHipNumCUs();
+// Blocks per compute unit that the z-marching launches aim for (the
+// filter gradient's inner loop waits on LDS between barriers and wants more
+// resident waves than the convolution: 8 -> 8 at level 1 takes 480 us with
+// 4 per CU against 693 with 2; the convolutions are fastest with 2).
+constexpr int kConvZBlocksPerCU = 2;
+constexpr int kWgradZBlocksPerCU = 4;
+
+// z chunks of a z-marching launch: enough blocks for per_cu per compute
+// unit over `tiles` xy tiles of N problems of D planes.
+void ZChunks(long tiles, int N, int D, int per_cu, int *tz, int *zchunk) {
+  const long want = (long)per_cu * __PSHipNumCUs();
   long t = (want + tiles * N - 1) / (tiles * N);
   if (t < 1) t = 1;
   if (t > D) t = D;
@@ -1123,7 +1200,7 @@ void ConvZLaunch(const u16 *x, const u16 *wt, const float *b, bool relu,
                  u16 *y, int N, int D, int H, int W) {
   const int tiles = ((W + XT - 1) / XT) * ((H + TY - 1) / TY);
   int tz, zchunk;
-  ZChunks(tiles, N, D, &tz, &zchunk);
+  ZChunks(tiles, N, D, kConvZBlocksPerCU, &tz, &zchunk);
   const dim3 grid(tiles, N * tz), block(64 * TY);
   if (relu)
     hipLaunchKernelGGL((ConvZKernel<CIN, COUT, true, XT, TY>), grid, block, 0,
@@ -1171,7 +1248,7 @@ long WgradZBlocks(int cin, int N, int D, int H, int W, int *tz, int *zchunk) {
   const int xt = WgradZXT(cin, W);
   const long tiles = (long)((W + xt - 1) / xt) *
                      ((H + kWgradZRows - 1) / kWgradZRows);
-  ZChunks(tiles, N, D, tz, zchunk);
+  ZChunks(tiles, N, D, kWgradZBlocksPerCU, tz, zchunk);
   return tiles * N * *tz;
 }
 
@@ -1314,6 +1391,9 @@ struct __PSUNet {
   float *part = nullptr, *bpart = nullptr, *tmp = nullptr;
   double *lpart = nullptr, *lhost = nullptr;
   size_t part_elems = 0, bpart_elems = 0;
+  // smooth-error loss weights and the cell means of the error
+  float w4 = 0.0f, w16 = 0.0f;
+  float *e4 = nullptr, *e16 = nullptr;
 };
 
 namespace {
@@ -1443,9 +1523,25 @@ double Backward(__PSUNet *net, const __PSGrid *target, int first, int count) {
   if (target->type != PS_FLOAT) PS_FATAL("U-Net: target must be a float grid");
   const Dims d = DimsOf(net, count);
   const long off = (long)first * net->nx * net->ny * net->nzs;
-  hipLaunchKernelGGL(LossKernel, dim3(kLossBlocks), dim3(256), 0, Stream(),
-                     net->out, (const float *)target->p, off,
-                     1.0f / net->out_scale, net->dout, net->lpart, d.P1);
+  const float inv_scale = 1.0f / net->out_scale;
+  if (net->w4 != 0.0f || net->w16 != 0.0f) {
+    hipLaunchKernelGGL(SmoothCellKernel, dim3(Blocks(d.P2 * 64)), dim3(256),
+                       0, Stream(), net->out, (const float *)target->p, off,
+                       inv_scale, net->e4, d.D1, d.H1, d.W1, d.P2);
+    Check();
+    hipLaunchKernelGGL(SmoothCoarseKernel, dim3(Blocks(d.P3)), dim3(256), 0,
+                       Stream(), net->e4, net->e16, d.D2, d.H2, d.W2, d.P3);
+    Check();
+    hipLaunchKernelGGL(LossKernel<true>, dim3(kLossBlocks), dim3(256), 0,
+                       Stream(), net->out, (const float *)target->p, off,
+                       inv_scale, net->e4, net->e16, net->w4, net->w16, d.D1,
+                       d.H1, d.W1, net->dout, net->lpart, d.P1);
+  } else {
+    hipLaunchKernelGGL(LossKernel<false>, dim3(kLossBlocks), dim3(256), 0,
+                       Stream(), net->out, (const float *)target->p, off,
+                       inv_scale, nullptr, nullptr, 0.0f, 0.0f, d.D1, d.H1,
+                       d.W1, net->dout, net->lpart, d.P1);
+  }
   Check();
   // head: g1a = gated dD1 (layer 9's dz), head partials in part
   hipLaunchKernelGGL(HeadBackKernel, dim3(kHeadBlocks), dim3(256), 0,
@@ -1516,7 +1612,8 @@ void FreeNet(__PSUNet *net) {
                               net->c2, net->c1, net->out, net->dout, net->g1a,
                               net->g1b, net->gc1, net->g2a, net->g2b,
                               net->gc2, net->g3a, net->g3b, net->part,
-                              net->bpart, net->tmp, net->lpart};
+                              net->bpart, net->tmp, net->lpart, net->e4,
+                              net->e16};
   for (int i = 0; i < 10; ++i) ptrs.push_back(net->a[i]);
   for (auto &L : net->layers) {
     ptrs.push_back(L.wt);
@@ -1653,6 +1750,8 @@ __PSUNet *PSUNetNew(int nx, int ny, int nzs, int max_batch, float out_scale,
   net->part = A32((long)pe);
   net->bpart = A32((long)be);
   net->tmp = A32((long)(te * kSumSegments));
+  net->e4 = A32(d.P2);
+  net->e16 = A32(d.P3);
   net->lpart = (double *)Alloc(sizeof(double) * kLossBlocks);
   ok = ok && net->lpart;
   if (ok && hipHostMalloc((void **)&net->lhost, sizeof(double) * kLossBlocks) !=
@@ -1744,6 +1843,13 @@ double PSUNetLossGrad(__PSUNet *net, const __PSGrid *rhs, const __PSGrid *bnd,
     PS_HIP_CHECK(hipMemcpy(grad_host, net->grad, sizeof(float) * net->nparams,
                            hipMemcpyDeviceToHost));
   return loss;
+}
+
+void PSUNetSetLossWeights(__PSUNet *net, float w4, float w16) {
+  if (!(w4 >= 0.0f) || !(w16 >= 0.0f))
+    PS_FATAL("U-Net: loss weights must be >= 0 (got %g, %g)", w4, w16);
+  net->w4 = w4;
+  net->w16 = w16;
 }
 
 double PSUNetTrainStep(__PSUNet *net, const __PSGrid *rhs,

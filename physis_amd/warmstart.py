@@ -44,6 +44,7 @@ def library() -> C.CDLL:
         L.PSUNetTrainStep.argtypes = [C.c_void_p, G, G, G, C.c_int, C.c_int,
                                       C.c_float]
         L.PSUNetTrainStep.restype = C.c_double
+        L.PSUNetSetLossWeights.argtypes = [C.c_void_p, C.c_float, C.c_float]
         L.PSUNetReflect.argtypes = [G, C.c_int, C.c_int, G, C.c_int,
                                     C.c_int, C.c_int, C.c_int, C.c_int]
         L.PSConv3dForwardHost.argtypes = [U16P, U16P, F32P, C.c_int] + \
@@ -96,6 +97,11 @@ class UNet:
     def predict(self, rhs, bnd, first, count, p) -> None:
         self.L.PSUNetPredict(self.handle, rhs.handle, bnd.handle, first,
                              count, p.handle)
+
+    def set_loss_weights(self, w4: float, w16: float) -> None:
+        """Weights of the smooth-error terms (the error's means over 4^3 and
+        16^3 cells) in the training loss; 0, 0 is the mean square error."""
+        self.L.PSUNetSetLossWeights(self.handle, w4, w16)
 
     def loss_grad(self, rhs, bnd, target, first, count, want_grad=True):
         g = np.empty(self.num_params, np.float32) if want_grad else None

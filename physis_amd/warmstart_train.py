@@ -3,7 +3,7 @@ coefficients with 5% seeded noise and a3 balancing them, right-hand sides of
 4-24 Gaussians with sigma 2-16 cells) and writes its parameters.
 
     python -m physis_amd.warmstart_train [--ntrain 512] [--steps 6000]
-        [--out physis_amd/weights/unet128.npz]
+        [--smooth W4,W16] [--out physis_amd/weights/unet128.npz]
 
 Training data: ntrain problems (generator seeds 1000, 1064, ...; 64 per
 batched solve) solved by the BF16 refinement to relative residual 1e-5.
@@ -11,7 +11,9 @@ Each solved problem also yields its seven reflections (PSUNetReflect: the
 reflected right-hand side and solution solve the reflected problem, whose
 coefficients are drawn from the same distribution), so there are
 8 x ntrain pairs without more solves.  Adam over random windows of `batch`
-problems of one random reflection, lr decaying linearly from 1e-3 to 1e-4.
+problems of one random reflection, lr decaying linearly from 1e-3 to 1e-4,
+on the mean square error plus the smooth-error terms weighted W4, W16
+(PSUNetLossGrad).
 
 Writes OUT (numpy arrays: params float32, out_scale, n) and OUT with the
 suffix .json (sizes, loss and the cost of the solves and of the training).
@@ -72,27 +74,32 @@ def solve_training_set(ntrain: int, tol: float = 1e-5, chunk: int = 64):
     return rhs, bnd, sol, secs
 
 
-def train(ntrain: int, steps: int, batch: int, out: str, seed: int = 1,
-          lr: float = 1e-3, lr_final: float = 1e-4) -> dict:
-    from .warmstart import UNet, reflect
-    t0 = time.perf_counter()
-    rhs, bnd, sol, solve_s = solve_training_set(ntrain)
-    data_s = time.perf_counter() - t0
-    scale = float(np.max(np.abs(sol)))
+def training_grids(rhs, bnd, sol, ntrain: int):
+    """Device grids of the training set: rg[r], tg[r] the right-hand sides
+    and solutions under reflection r (0 = as solved), and bnd, which every
+    reflection leaves unchanged."""
+    from .warmstart import reflect
     shape = (N, N, N * ntrain)
-    # reflection r of the training set: grids rg[r], tg[r]; bnd is
-    # symmetric under every reflection
     rg = [_grid(rhs, shape)]
     tg = [_grid(sol, shape)]
     bg = _grid(bnd, shape)
-    del rhs, sol, bnd
     for r in range(1, 8):
         a, b = ps.PSGridNew(shape), ps.PSGridNew(shape)
         reflect(rg[0], 0, ntrain, a, 0, (N, N, N), r)
         reflect(tg[0], 0, ntrain, b, 0, (N, N, N), r)
         rg.append(a)
         tg.append(b)
+    return rg, tg, bg
+
+
+def fit(rg, tg, bg, ntrain: int, scale: float, steps: int, batch: int,
+        seed: int = 1, lr: float = 1e-3, lr_final: float = 1e-4,
+        w4: float = 0.0, w16: float = 0.0):
+    """Adam over random windows of `batch` problems of one random
+    reflection; returns (params, loss history, seconds)."""
+    from .warmstart import UNet
     net = UNet(N, N, N, batch, scale, seed=seed)
+    net.set_loss_weights(w4, w16)
     rng = np.random.default_rng(seed)
     hist = []
     t1 = time.perf_counter()
@@ -104,9 +111,23 @@ def train(ntrain: int, steps: int, batch: int, out: str, seed: int = 1,
         if s % max(steps // 20, 1) == 0 or s == steps - 1:
             print(f"[train] step {s}: loss {hist[-1]:.4e} "
                   f"({time.perf_counter() - t1:.1f} s)", flush=True)
-    train_s = time.perf_counter() - t1
+    secs = time.perf_counter() - t1
     params = net.get_params()
     net.free()
+    return params, hist, secs
+
+
+def train(ntrain: int, steps: int, batch: int, out: str, seed: int = 1,
+          lr: float = 1e-3, lr_final: float = 1e-4, w4: float = 0.0,
+          w16: float = 0.0) -> dict:
+    t0 = time.perf_counter()
+    rhs, bnd, sol, solve_s = solve_training_set(ntrain)
+    data_s = time.perf_counter() - t0
+    scale = float(np.max(np.abs(sol)))
+    rg, tg, bg = training_grids(rhs, bnd, sol, ntrain)
+    del rhs, sol, bnd
+    params, hist, train_s = fit(rg, tg, bg, ntrain, scale, steps, batch,
+                                seed, lr, lr_final, w4, w16)
     for g in rg + tg + [bg]:
         ps.PSGridFree(g)
     meta = {"problem": "128^3, face coefficients +-5%, a3 = 1/sum, "
@@ -114,6 +135,7 @@ def train(ntrain: int, steps: int, batch: int, out: str, seed: int = 1,
             "ntrain_solved": ntrain, "pairs": 8 * ntrain,
             "augmentation": "the 8 axis reflections of each solved problem",
             "steps": steps, "batch": batch, "lr": [lr, lr_final],
+            "loss": {"w4": w4, "w16": w16},
             "params": int(params.size), "out_scale": scale,
             "solve_seconds": round(solve_s, 2),
             "data_seconds": round(data_s, 2),
@@ -135,9 +157,12 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--smooth", default="0,0",
+                    help="W4,W16: weights of the smooth-error loss terms")
     a = ap.parse_args(argv)
+    w4, w16 = (float(v) for v in a.smooth.split(","))
     ps.PSInit()
-    train(a.ntrain, a.steps, a.batch, a.out)
+    train(a.ntrain, a.steps, a.batch, a.out, w4=w4, w16=w16)
     ps.PSFinalize()
     return 0
 

@@ -127,7 +127,10 @@ def _problem_grids(ps, coefs, shape):
     return rhs, bnd
 
 
-def test_network_loss_and_gradient(ps):
+@pytest.mark.parametrize("w4,w16", [(0.0, 0.0), (4.0, 16.0)])
+def test_network_loss_and_gradient(ps, w4, w16):
+    """Whole-network loss and gradient against the oracle, with the mean
+    square error and with the smooth-error terms."""
     from physis_amd.warmstart import UNet
     nx = ny = nzs = 32
     nb, scale = 2, 4.0
@@ -144,16 +147,19 @@ def test_network_loss_and_gradient(ps):
     try:
         params = net.get_params()
         assert params.size == U.num_params()
+        net.set_loss_weights(w4, w16)
         loss, grad = net.loss_grad(rhs, bnd, tg, 0, nb)
         rloss, rgrad = U.loss_grad(params, coefs[11], coefs[10], target,
-                                   (nx, ny, nzs), nb, scale)
+                                   (nx, ny, nzs), nb, scale, w4=w4, w16=w16)
         print("loss", loss, rloss, "rel", abs(loss - rloss) / rloss,
               flush=True)
-        assert abs(loss - rloss) <= 1e-2 * rloss
+        # measured: loss 2.4e-5 relative, gradient 1.7e-3 relative L2 (BF16
+        # activations in a different summation order); bounds about 5x
+        assert abs(loss - rloss) <= 1.5e-4 * rloss
         err = np.linalg.norm(grad - rgrad) / np.linalg.norm(rgrad)
         cos = float(grad @ rgrad / np.linalg.norm(grad) / np.linalg.norm(rgrad))
         print("grad rel err", err, "cos", cos, flush=True)
-        assert err <= 5e-2 and cos >= 0.998
+        assert err <= 1e-2 and cos >= 0.9999
         # every layer's slice agrees too (a wrong layer cannot hide)
         off = 0
         for ci, co in U.LAYERS:
@@ -161,7 +167,7 @@ def test_network_loss_and_gradient(ps):
             g, r = grad[off:off + n], rgrad[off:off + n]
             lerr = np.linalg.norm(g - r) / np.linalg.norm(r)
             print(f"layer {ci}->{co}: grad rel err {lerr:.3e}", flush=True)
-            assert lerr <= 0.1, (ci, co)
+            assert lerr <= 1.5e-2, (ci, co)  # measured <= 2.5e-3
             off += n
         # prediction = out_scale * out * bnd
         pg = ps.PSGridNew(shape)
@@ -178,10 +184,13 @@ def test_network_loss_and_gradient(ps):
 
 
 def test_warm_start_saves_sweeps(ps):
-    """Train on Jacobi solutions of 64 problems, then solve 8 unseen ones from
-    the prediction and from zero: the warm start needs fewer sweeps."""
+    """Train on Jacobi solutions of 64 problems and their 8 axis
+    reflections, then solve 8 unseen ones from the prediction and from zero:
+    the warm start needs at least 10% fewer sweeps (measured 12.7%; the
+    training is deterministic).  The committed 128^3 network's saving of
+    at least 15% is tested in test_gpu_warmstart128.py."""
     from physis_amd.poisson import PoissonSolver
-    from physis_amd.warmstart import UNet
+    from physis_amd.warmstart import UNet, reflect
     nx = ny = nzs = 32
     ntrain, ntest = 64, 8
     nb = ntrain + ntest
@@ -205,9 +214,17 @@ def test_warm_start_saves_sweeps(ps):
         tgt = ps.PSGridNew(shape)
         ps.PSGridCopyin(tgt, sol)
         scale = float(np.max(np.abs(sol)))
+        # the training problems under the 8 reflections, stacked
+        ashape = (nx, ny, nzs * ntrain * 8)
+        aug = [ps.PSGridNew(ashape) for _ in range(3)]
+        for r in range(8):
+            for src, dst in zip((grids[11], grids[10], tgt), aug):
+                reflect(src, 0, ntrain, dst, r * ntrain, (nx, ny, nzs), r)
         net = UNet(nx, ny, nzs, 8, scale, seed=3)
-        hist = net.train(grids[11], grids[10], tgt, ntrain, steps=800,
-                         batch=8, lr=1e-3, log=100)
+        hist = net.train(aug[0], aug[1], aug[2], ntrain * 8, steps=3000,
+                         batch=8, lr=1e-3, lr_final=1e-4, log=500)
+        for g in aug:
+            ps.PSGridFree(g)
         first, last = np.mean(hist[:10]), np.mean(hist[-50:])
         print("loss first/last", first, last, flush=True)
         assert last < 0.5 * first
@@ -230,7 +247,9 @@ def test_warm_start_saves_sweeps(ps):
         print("zero start", zero, flush=True)
         print("warm start", warm, flush=True)
         assert zero["converged"] and warm["converged"]
-        assert warm["sweeps"] < zero["sweeps"]
+        saving = 1.0 - warm["sweeps"] / zero["sweeps"]
+        print("sweep saving", saving, flush=True)
+        assert saving >= 0.10
         ts.free()
         net.free()
         for g in tgrids + [pz, pw, tgt]:

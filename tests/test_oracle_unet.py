@@ -110,3 +110,56 @@ def test_network_gradient_matches_finite_differences():
                             torch.float64)
         fd = (lp - lm) / (2 * eps)
         assert abs(fd - grad[i]) <= 1e-6 + 1e-4 * abs(fd), (i, fd, grad[i])
+
+
+@pytest.mark.parametrize("w4,w16", [(4.0, 16.0), (0.5, 0.0), (0.0, 2.0)])
+def test_smooth_loss_terms_closed_forms(w4, w16):
+    """The smooth-error terms of the loss on errors whose cell means are
+    known: a constant error c has every cell mean c (loss (1 + w4 + w16)
+    c^2); an error c (-1)^(x+y+z) has every 4^3 and 16^3 cell mean 0
+    (loss c^2); an error c (-1)^(floor(x/4)) has 4^3 cell means +-c and
+    16^3 cell means 0 (loss (1 + w4) c^2).  A wrong cell size, a sum in
+    place of a mean or a term dropped fails one of them."""
+    shape, nb, scale = (32, 16, 16), 2, 2.0
+    nx, ny, nzs = shape
+    rng = np.random.default_rng(5)
+    n = nx * ny * nzs * nb
+    params = rng.standard_normal(U.num_params()) * 0.1
+    rhs = rng.standard_normal(n)
+    bnd = np.ones(n)
+    out, _ = U.forward(params, rhs, bnd, shape, nb, scale,
+                       rounding=False, dtype=torch.float64)
+    out = out.detach().numpy().reshape(-1)
+    z, y, x = np.meshgrid(np.arange(nzs * nb), np.arange(ny), np.arange(nx),
+                          indexing="ij")
+    c = 0.25
+    for err, want in ((np.full(n, c), (1 + w4 + w16) * c * c),
+                      (c * (-1.0) ** (x + y + z).reshape(-1), c * c),
+                      (c * (-1.0) ** (x // 4).reshape(-1), (1 + w4) * c * c)):
+        target = (out - err) * scale  # d = out - target / scale = err
+        loss, _ = U.loss_grad(params, rhs, bnd, target, shape, nb, scale,
+                              rounding=False, dtype=torch.float64, w4=w4,
+                              w16=w16)
+        assert abs(loss - want) <= 1e-12 * want, (loss, want)
+
+
+def test_smooth_loss_gradient_matches_finite_differences():
+    shape, nb, scale = (16, 16, 16), 1, 3.0
+    rng = np.random.default_rng(3)
+    n = int(np.prod(shape))
+    params = rng.standard_normal(U.num_params()) * 0.15
+    rhs = rng.standard_normal(n)
+    bnd = np.ones(n)
+    target = rng.standard_normal(n) + 2.0
+    kw = dict(rounding=False, dtype=torch.float64, w4=3.0, w16=5.0)
+    _, grad = U.loss_grad(params, rhs, bnd, target, shape, nb, scale, **kw)
+    last = U.num_params() - 1
+    eps = 1e-5
+    for i in (3, 1000, last - 4, last):
+        pp, pm = params.copy(), params.copy()
+        pp[i] += eps
+        pm[i] -= eps
+        lp, _ = U.loss_grad(pp, rhs, bnd, target, shape, nb, scale, **kw)
+        lm, _ = U.loss_grad(pm, rhs, bnd, target, shape, nb, scale, **kw)
+        fd = (lp - lm) / (2 * eps)
+        assert abs(fd - grad[i]) <= 1e-6 + 1e-4 * abs(fd), (i, fd, grad[i])
