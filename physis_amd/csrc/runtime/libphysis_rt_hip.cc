@@ -35,6 +35,7 @@ void EnsureInit() {
   r.device = dev;
   r.num_cus = prop.multiProcessorCount;
   r.stream = nullptr;
+  InitNoAllocScratch(dev);
   r.initialized = true;
 }
 
@@ -371,6 +372,10 @@ void __PSReduceGridInt(void *buf, enum PSReduceOp op, __PSGrid *g) {
 }
 void __PSReduceGridLong(void *buf, enum PSReduceOp op, __PSGrid *g) {
   ReduceGrid(buf, op, g, PS_LONG);
+}
+void __PSReduceGridNoAlloc(void *buf, enum PSReduceOp op, __PSGrid *g,
+                           PSType type) {
+  ReduceGridNoAlloc(buf, op, g, type);
 }
 
 void *__PSHipStream(void) {

@@ -101,16 +101,23 @@ __global__ __launch_bounds__(1024) void ReduceStage2(
   if (threadIdx.x == 0) out[0] = (T)r;
 }
 
-template <typename T, int OP>
-void Launch(const T *p, int64_t n, void *host_out) {
-  Runtime &r = rt();
+// Number of stage-1 blocks for n elements: enough to stream the grid, at
+// most eight per compute unit.
+template <typename T>
+int StageBlocks(int64_t n) {
   int64_t per_block = (int64_t)kThreads * Vec16<T>::N * kUnroll;
   int64_t want = (n + per_block - 1) / per_block;
-  int64_t cap = (int64_t)r.num_cus * 8;
-  int nb = (int)(want < 1 ? 1 : (want > cap ? cap : want));
-  typedef typename Acc<T, OP>::A A;
-  A *parts = (A *)Scratch(0, sizeof(A) * (size_t)(nb + 1));
-  T *out = (T *)(parts + nb);
+  int64_t cap = (int64_t)rt().num_cus * 8;
+  return (int)(want < 1 ? 1 : (want > cap ? cap : want));
+}
+
+// Both stages on the runtime stream: nb partials into `parts`, the result
+// (one T) into `out`; then the result is copied to host_out and the stream
+// drained.
+template <typename T, int OP>
+void LaunchStages(const T *p, int64_t n, int nb,
+                  typename Acc<T, OP>::A *parts, T *out, void *host_out) {
+  Runtime &r = rt();
   hipLaunchKernelGGL((ReduceStage1<T, OP>), dim3(nb), dim3(kThreads), 0,
                      r.stream, p, n, parts);
   PS_HIP_CHECK(hipGetLastError());
@@ -120,6 +127,14 @@ void Launch(const T *p, int64_t n, void *host_out) {
   PS_HIP_CHECK(hipMemcpyAsync(host_out, out, sizeof(T), hipMemcpyDeviceToHost,
                               r.stream));
   PS_HIP_CHECK(hipStreamSynchronize(r.stream));
+}
+
+template <typename T, int OP>
+void Launch(const T *p, int64_t n, void *host_out) {
+  const int nb = StageBlocks<T>(n);
+  typedef typename Acc<T, OP>::A A;
+  A *parts = (A *)Scratch(0, sizeof(A) * (size_t)(nb + 1));
+  LaunchStages<T, OP>(p, n, nb, parts, (T *)(parts + nb), host_out);
 }
 
 template <typename T>
@@ -137,10 +152,63 @@ void Dispatch(void *buf, PSReduceOp op, const __PSGrid *g) {
   std::memcpy(buf, slot, sizeof(T));
 }
 
-}  // namespace
+// ------------------------------------------- reduction without allocation
+// Partials and result of __PSReduceGridNoAlloc live in this file-scope
+// device array, one copy per GPU, held until the process ends.  Host code
+// never passes `scratch` itself to a kernel or copy (in host code the name
+// is a stand-in, not a device address): it uses the address that
+// hipGetSymbolAddress returns for the current device, kept in g_scratch_addr.
+// Stage 1 writes every partial it launches and stage 2 writes the result,
+// so nothing in the array has to start at zero.
+constexpr int kNoAllocMaxParts = 4096;
+constexpr size_t kNoAllocScratchBytes = 8 * (kNoAllocMaxParts + 1);
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
-void ReduceGrid(void *buf, PSReduceOp op, const __PSGrid *g, PSType type) {
-  EnsureInit();
+constexpr int kMaxDevices = 64;
+void *g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on device `dev`, looked up once per device.
+void *NoAllocScratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices)
+    PS_FATAL("device %d beyond the %d devices the no-allocation reduction "
+             "supports", dev, kMaxDevices);
+  if (!g_scratch_addr[dev]) {
+    void *ptr = nullptr;
+    PS_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) PS_FATAL("no device address for the reduction scratch array");
+    g_scratch_addr[dev] = ptr;
+  }
+  return g_scratch_addr[dev];
+}
+
+template <typename T, int OP>
+void LaunchNoAlloc(const T *p, int64_t n, void *buf, unsigned char *area) {
+  typedef typename Acc<T, OP>::A A;
+  int nb = StageBlocks<T>(n);
+  if (nb > kNoAllocMaxParts) nb = kNoAllocMaxParts;
+  const size_t need = sizeof(A) * (size_t)nb + sizeof(T);
+  if (need > kNoAllocScratchBytes)
+    PS_FATAL("reduction needs %zu scratch bytes, more than the %zu of the "
+             "no-allocation area", need, kNoAllocScratchBytes);
+  A *parts = (A *)area;
+  LaunchStages<T, OP>(p, n, nb, parts, (T *)(parts + nb), buf);
+}
+
+template <typename T>
+void DispatchNoAlloc(void *buf, PSReduceOp op, const __PSGrid *g,
+                     unsigned char *area) {
+  const T *p = (const T *)g->p;
+  switch (op) {
+    case PS_MAX: LaunchNoAlloc<T, PS_MAX>(p, g->num_elms, buf, area); break;
+    case PS_MIN: LaunchNoAlloc<T, PS_MIN>(p, g->num_elms, buf, area); break;
+    case PS_SUM: LaunchNoAlloc<T, PS_SUM>(p, g->num_elms, buf, area); break;
+    case PS_PROD: LaunchNoAlloc<T, PS_PROD>(p, g->num_elms, buf, area); break;
+    default: PS_FATAL("unknown reduction operator %d", (int)op);
+  }
+}
+
+// Checks that grid g can be reduced as `type`.
+void CheckReducible(const __PSGrid *g, PSType type) {
   int want_size = 0;
   switch (type) {
     case PS_FLOAT: want_size = sizeof(float); break;
@@ -149,15 +217,44 @@ void ReduceGrid(void *buf, PSReduceOp op, const __PSGrid *g, PSType type) {
     case PS_LONG: want_size = sizeof(long); break;
     default: PS_FATAL("PSReduce on unsupported element type %d", (int)type);
   }
+  if (!g) PS_FATAL("PSReduce on a null grid");
   if (g->user) PS_FATAL("PSReduce on a grid of user-defined points");
   if (g->elm_size != want_size)
     PS_FATAL("PSReduce: grid element size %d does not match the %d-byte "
              "reduction type", g->elm_size, want_size);
+  if (g->num_elms <= 0) PS_FATAL("PSReduce on an empty grid");
+}
+
+}  // namespace
+
+void InitNoAllocScratch(int dev) { NoAllocScratch(dev); }
+
+void ReduceGrid(void *buf, PSReduceOp op, const __PSGrid *g, PSType type) {
+  EnsureInit();
+  CheckReducible(g, type);
   switch (type) {
     case PS_FLOAT: Dispatch<float>(buf, op, g); break;
     case PS_DOUBLE: Dispatch<double>(buf, op, g); break;
     case PS_INT: Dispatch<int>(buf, op, g); break;
     case PS_LONG: Dispatch<long>(buf, op, g); break;
+    default: break;
+  }
+}
+
+void ReduceGridNoAlloc(void *buf, PSReduceOp op, const __PSGrid *g,
+                       PSType type) {
+  if (!rt().initialized)
+    PS_FATAL("__PSReduceGridNoAlloc before PSInit (the runtime must be set "
+             "up first)");
+  CheckReducible(g, type);
+  int dev = 0;
+  PS_HIP_CHECK(hipGetDevice(&dev));
+  unsigned char *area = (unsigned char *)NoAllocScratch(dev);
+  switch (type) {
+    case PS_FLOAT: DispatchNoAlloc<float>(buf, op, g, area); break;
+    case PS_DOUBLE: DispatchNoAlloc<double>(buf, op, g, area); break;
+    case PS_INT: DispatchNoAlloc<int>(buf, op, g, area); break;
+    case PS_LONG: DispatchNoAlloc<long>(buf, op, g, area); break;
     default: break;
   }
 }

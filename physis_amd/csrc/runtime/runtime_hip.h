@@ -53,6 +53,12 @@ void *HostResultSlot();
 
 // reduce_hip.hip
 void ReduceGrid(void *buf, PSReduceOp op, const __PSGrid *g, PSType type);
+// The same reduction with its partials in a fixed device array (no
+// allocation); InitNoAllocScratch looks up that array's address on device
+// dev and is called once per device at set-up.
+void ReduceGridNoAlloc(void *buf, PSReduceOp op, const __PSGrid *g,
+                       PSType type);
+void InitNoAllocScratch(int dev);
 
 // Storage of a grid of user-defined points (type PS_USER with members):
 // member by member, each scalar component of a member (array members have
