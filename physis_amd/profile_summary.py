@@ -3,6 +3,8 @@
     python3 -m physis_amd.profile_summary FETCH_DIR WRITE_DIR KTRACE_DIR OUT.md
     python3 -m physis_amd.profile_summary --ktrace KTRACE_DIR OUT.md COMMAND
     python3 -m physis_amd.profile_summary --unet KTRACE_DIR OUT.md COMMAND
+    python3 -m physis_amd.profile_summary --counters DIR... OUT.md COMMAND
+    python3 -m physis_amd.profile_summary --counters-unet DIR... OUT.md COMMAND
 
 The first form summarises physis_amd.profile_kernels (HBM traffic of the
 hot-path kernels); --ktrace tabulates the kernel statistics of any command;
@@ -236,17 +238,28 @@ def unet_md(kt_dir: str, out_md: str, command: str) -> None:
     print("\n".join(lines))
 
 
-def counters_md(dirs, out_md: str, command: str) -> None:
+STENCIL_FRAGS = [
+    ("two-sweep 7-pt (clamped)", "__PSStencilRun2_diffusion7_kernel<"),
+    ("one-sweep 7-pt (clamped)", "__PSStencilRun_diffusion7_kernel<"),
+    ("two-sweep 7-pt (periodic)", "__PSStencilRun2_diffusion7_periodic_kernel<"),
+    ("512 MiB copy", "copyBuffer")]
+# the level-1 kernels of physis_amd.profile_unet (batch 8 of 128^3)
+UNET_FRAGS = [
+    ("forward 8->8", "ConvZKernel<8, 8, true"),
+    ("forward 24->8", "ConvZKernel<24, 8, true"),
+    ("backward data 8->24", "ConvZKernel<24, 8, false"),
+    ("backward filter 8->8", "WgradZKernel<8, 8,"),
+    ("backward filter 24->8", "WgradZKernel<24, 8,")]
+
+
+def counters_md(dirs, out_md: str, command: str, frags=None,
+                title="the 7-point run kernels") -> None:
     """Per-launch averages of the instruction / stall counters of
     `rocprofv3 --pmc` runs (one directory per counter group) for the
-    7-point run kernels, with derived rates: busy and wait fractions of the
-    wave cycles, instructions per wave, LDS bank conflicts per LDS
-    instruction."""
-    frags = [("two-sweep 7-pt (clamped)", "__PSStencilRun2_diffusion7_kernel<"),
-             ("one-sweep 7-pt (clamped)", "__PSStencilRun_diffusion7_kernel<"),
-             ("two-sweep 7-pt (periodic)",
-              "__PSStencilRun2_diffusion7_periodic_kernel<"),
-             ("512 MiB copy", "copyBuffer")]
+    kernels of `frags` (label, kernel-name fragment), with derived rates:
+    busy and wait fractions of the wave cycles, instructions per wave, LDS
+    bank conflicts per LDS instruction."""
+    frags = STENCIL_FRAGS if frags is None else frags
     vals = defaultdict(dict)
     for d in dirs:
         for f in glob.glob(os.path.join(d, "**", "*_counter_collection.csv"),
@@ -261,8 +274,8 @@ def counters_md(dirs, out_md: str, command: str) -> None:
             for (label, c), v in acc.items():
                 vals[label][c] = sum(v) / len(v)
     names = sorted({c for v in vals.values() for c in v})
-    lines = ["# Instruction and stall counters of the 7-point run kernels "
-             "(MI355X)", "", "```", command, "```", "",
+    lines = [f"# Instruction and stall counters of {title} (MI355X)", "",
+             "```", command, "```", "",
              "Per-launch averages (rocprofv3 sums each counter over the "
              "device for a dispatch).", "",
              "| counter | " + " | ".join(l for l, _ in frags if l in vals) + " |",
@@ -287,7 +300,8 @@ def counters_md(dirs, out_md: str, command: str) -> None:
                     parts.append(f"{k} / SQ_WAVE_CYCLES = {v[k] / wc:.3f}")
         if waves:
             for k in ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS",
-                      "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR"):
+                      "SQ_INSTS_MFMA", "SQ_INSTS_VMEM_RD",
+                      "SQ_INSTS_VMEM_WR"):
                 if k in v:
                     parts.append(f"{k} per wave = {v[k] / waves:.1f}")
         if v.get("SQ_LDS_BANK_CONFLICT") is not None and \
@@ -303,6 +317,10 @@ def counters_md(dirs, out_md: str, command: str) -> None:
 if __name__ == "__main__":
     if len(sys.argv) >= 5 and sys.argv[1] == "--counters":
         counters_md(sys.argv[2:-2], sys.argv[-2], sys.argv[-1])
+        sys.exit(0)
+    if len(sys.argv) >= 5 and sys.argv[1] == "--counters-unet":
+        counters_md(sys.argv[2:-2], sys.argv[-2], sys.argv[-1], UNET_FRAGS,
+                    "the level-1 U-Net convolution kernels")
         sys.exit(0)
     if len(sys.argv) == 5 and sys.argv[1] == "--unet":
         unet_md(*sys.argv[2:])
