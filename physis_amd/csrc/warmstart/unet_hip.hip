@@ -321,38 +321,56 @@ __global__ __launch_bounds__(64 * TY) void ConvZKernel(
   }
 }
 
-// Filter gradient partials marching z: block (x tile, y tile, problem, z
-// chunk) sums over its XT x TY x zchunk positions
+// Filter gradient partials marching the input planes: block (x tile, y
+// tile, problem, z chunk [zb, ze) of output-gradient planes) sums over its
+// XT x TY x zchunk positions
 //   part[block][co][k] = sum_p dz[p][co] x[p + off(k)][ci],  k = tap CIN + ci
-// as a GEMM over positions (K) on v_mfma_f32_16x16x32_bf16: A = dz^T (a
-// lane's 8 positions of one channel, staged transposed dzT[co][y][x]), B =
-// the shifted input (8 consecutive x of one channel, staged transposed
-// xT[plane][row][ci][x] in a ring of 4 planes, read as 32-bit words with a
-// funnel shift for odd offsets; one unaligned 16-byte LDS read instead
-// measured 1.8-2.4x slower).  Wave w takes column tiles w, w + TY, ...
-// against the A operands of every 32-position step of the plane; wave 0
-// also sums dz per channel for the bias.  The planes of the input are
-// staged once per tile.
-template <int CIN, int COUT, int XT, int TY>
+// as a GEMM over positions (K) on v_mfma_f32_16x16x32_bf16.  Step z' stages
+// input plane z' (transposed, xT[row][ci][x], 3 slots) and uses it for the
+// three taps whose z offset t - 1 maps it to output-gradient plane
+// z' + 1 - t (t = 0, 1, 2; planes outside [zb, ze) stage as zeros, dzT
+// [co][y][x] in a ring of 4).  B = the shifted input (8 consecutive x of
+// one channel at one (dy, dx) offset: column (3 dy + dx) CIN + ci, read as
+// 32-bit words with a funnel shift for odd offsets) is read once per step
+// and feeds every t: the A operand's rows pack (t, co) pairs, row
+// q = t COUT + co, so COUT = 8 needs 2 MFMAs per B operand (the tap-major
+// order needed 3, each half empty, and read B three times).  Waves split
+// into WT groups over column tiles and TY / WT groups over the 32-position
+// steps of a plane; the step groups' sums are added in a fixed order at
+// the end (deterministic).  One barrier per step.
+template <int CIN, int COUT, int XT, int TY, int WT>
 __global__ __launch_bounds__(64 * TY) void WgradZKernel(
     const u16 *__restrict__ x, const u16 *__restrict__ dz,
     float *__restrict__ part, float *__restrict__ bpart, int D, int H, int W,
     int tz, int zchunk) {
   static_assert(XT % 32 == 0, "steps of 32 positions along x");
-  constexpr int K = 27 * CIN, NTN = (K + 15) / 16, MT = (COUT + 15) / 16;
-  constexpr int TPW = (NTN + TY - 1) / TY;  // column tiles per wave
+  static_assert(TY % WT == 0, "wave groups");
+  constexpr int K = 27 * CIN, NC = 9 * CIN;
+  constexpr int NTN = (NC + 15) / 16;       // column tiles
+  constexpr int MM = (3 * COUT + 15) / 16;  // MFMAs per B operand
+  constexpr int WS = TY / WT;               // step groups
+  constexpr int TPW = (NTN + WT - 1) / WT;  // column tiles per wave
+  constexpr int NSTEP = TY * XT / 32;       // 32-position steps per plane
+  static_assert(NSTEP % WS == 0, "steps split evenly");
   constexpr int XR = XT + 2, RS = TY + 2;
   constexpr int RW = CIN * XR / 2;          // 32-bit words per staged row
   constexpr int PW = RS * RW;               // words per staged plane
   constexpr int NTH = 64 * TY;
   constexpr int XV = RS * XR * (CIN / 8);   // 16-byte input vectors per plane
   constexpr int XVT = (XV + NTH - 1) / NTH;
+  constexpr int DROW = TY * XT + 8;         // dzT row (one channel), padded
+  constexpr int DPL = COUT * DROW;          // u16 per staged dz plane
   constexpr int DV = TY * XT * (COUT / 8);  // 16-byte dz vectors per plane
   constexpr int DVT = (DV + NTH - 1) / NTH;
-  static_assert(XR % 2 == 0, "x pairs");
-  __shared__ unsigned xT[4 * PW + 4];
-  __shared__ __attribute__((aligned(16))) u16 dzT[COUT * TY * XT];
+  constexpr int LOOP_WORDS = 3 * PW + 4 + 4 * DPL / 2;
+  constexpr int RED_WORDS = (WS - 1) * WT * (TPW * MM * 4 + MM) * 64;
+  constexpr int LDS_WORDS = LOOP_WORDS > RED_WORDS ? LOOP_WORDS : RED_WORDS;
+  static_assert(XR % 2 == 0 && DROW % 8 == 0, "alignment");
+  __shared__ __attribute__((aligned(16))) unsigned lds[LDS_WORDS];
+  unsigned *const xT = lds;
+  u16 *const dzT = reinterpret_cast<u16 *>(lds + 3 * PW + 4);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wt = wv % WT, ws = wv / WT;
   const int g = lane >> 4, r = lane & 15;
   const int tiles_x = (W + XT - 1) / XT;
   const int x0 = (blockIdx.x % tiles_x) * XT, y0 = (blockIdx.x / tiles_x) * TY;
@@ -360,30 +378,36 @@ __global__ __launch_bounds__(64 * TY) void WgradZKernel(
   const int zb = (blockIdx.y - n * tz) * zchunk;
   const int ze = min(zb + zchunk, D);
   const long bid = (long)blockIdx.y * gridDim.x + blockIdx.x;
-  // word offset (within a plane) of each tile column's (row dy, ci, x dx)
-  int lofs[TPW], lpl[TPW];
+  // element offset (within a staged plane) of each tile column's (dy, ci,
+  // dx) origin; -1 past the last column
+  int lofs[TPW];
 #pragma unroll
   for (int i = 0; i < TPW; ++i) {
-    const int col = (wv + TY * i) * 16 + r;
-    if (col < K) {
-      const int tap = col / CIN, ci = col - tap * CIN;
-      lpl[i] = tap / 9;
-      lofs[i] = (((tap / 3) % 3) * CIN + ci) * XR + tap % 3;  // elements
+    const int col = (wt + WT * i) * 16 + r;
+    if (col < NC) {
+      const int o = col / CIN, ci = col - o * CIN;
+      lofs[i] = ((o / 3) * CIN + ci) * XR + o % 3;
     } else {
-      lpl[i] = 0;
       lofs[i] = -1;
     }
   }
-  f32x4 acc[TPW][MT];
+  // A rows of this lane: q = 16 m + r -> (t, co); t = 3 means none
+  int at[MM], aco[MM];
+#pragma unroll
+  for (int m = 0; m < MM; ++m) {
+    const int q = 16 * m + r;
+    at[m] = q < 3 * COUT ? q / COUT : 3;
+    aco[m] = q < 3 * COUT ? q % COUT : 0;
+  }
+  f32x4 acc[TPW][MM];
 #pragma unroll
   for (int i = 0; i < TPW; ++i)
 #pragma unroll
-    for (int m = 0; m < MT; ++m) acc[i][m] = f32x4{0.f, 0.f, 0.f, 0.f};
-  float bsum[MT];
+    for (int m = 0; m < MM; ++m) acc[i][m] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float bsum[MM];  // dz sums of the t = 1 rows (bias gradient)
 #pragma unroll
-  for (int m = 0; m < MT; ++m) bsum[m] = 0.0f;
-  // input plane zz, rows y0-1 .. y0+TY, x pairs: vector v -> (row, x pair
-  // half, channel group)
+  for (int m = 0; m < MM; ++m) bsum[m] = 0.0f;
+  // input plane zz, rows y0-1 .. y0+TY, x0-1 .. x0+XT
   auto fetchx = [&](int zz, uint4 (&buf)[XVT]) {
 #pragma unroll
     for (int i = 0; i < XVT; ++i) {
@@ -399,7 +423,7 @@ __global__ __launch_bounds__(64 * TY) void WgradZKernel(
       buf[i] = u;
     }
   };
-  // transposed into xT[slot][row][ci][x] (16-bit elements)
+  // transposed into slot `slot` of xT: [row][ci][x] (16-bit elements)
   auto putx = [&](int slot, const uint4 (&buf)[XVT]) {
     u16 *base = reinterpret_cast<u16 *>(xT + slot * PW);
 #pragma unroll
@@ -417,6 +441,7 @@ __global__ __launch_bounds__(64 * TY) void WgradZKernel(
       }
     }
   };
+  // output-gradient plane zz (zeros outside [zb, ze))
   auto fetchd = [&](int zz, uint4 (&buf)[DVT]) {
 #pragma unroll
     for (int i = 0; i < DVT; ++i) {
@@ -425,13 +450,14 @@ __global__ __launch_bounds__(64 * TY) void WgradZKernel(
       const int xp = t % XT, row = t / XT;
       const int yy = y0 + row, xx = x0 + xp;
       uint4 u = make_uint4(0, 0, 0, 0);
-      if (v < DV && zz < ze && yy < H && xx < W)
+      if (v < DV && zz >= zb && zz < ze && yy < H && xx < W)
         u = *reinterpret_cast<const uint4 *>(
             dz + ((((long)n * D + zz) * H + yy) * W + xx) * COUT + c8 * 8);
       buf[i] = u;
     }
   };
-  auto putd = [&](const uint4 (&buf)[DVT]) {
+  auto putd = [&](int slot, const uint4 (&buf)[DVT]) {
+    u16 *base = dzT + slot * DPL;
 #pragma unroll
     for (int i = 0; i < DVT; ++i) {
       const int v = threadIdx.x + i * NTH;
@@ -440,47 +466,52 @@ __global__ __launch_bounds__(64 * TY) void WgradZKernel(
       const unsigned w4[4] = {buf[i].x, buf[i].y, buf[i].z, buf[i].w};
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        dzT[(c8 * 8 + 2 * j) * (TY * XT) + t] = (u16)(w4[j] & 0xffffu);
-        dzT[(c8 * 8 + 2 * j + 1) * (TY * XT) + t] = (u16)(w4[j] >> 16);
+        base[(c8 * 8 + 2 * j) * DROW + t] = (u16)(w4[j] & 0xffffu);
+        base[(c8 * 8 + 2 * j + 1) * DROW + t] = (u16)(w4[j] >> 16);
       }
     }
   };
   if (zb < ze) {
+    // x plane zz in slot (zz - zb + 1) % 3, dz plane zz in (zz - zb + 2) & 3
     uint4 bx[XVT], bd[DVT];
+    fetchx(zb - 1, bx);
+    putx(0, bx);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      fetchx(zb - 1 + k, bx);
-      putx(k, bx);
+      fetchd(zb - 2 + k, bd);
+      putd(k, bd);
     }
-    fetchd(zb, bd);
-    putd(bd);
-    fetchx(zb + 2, bx);
+    fetchx(zb, bx);
     fetchd(zb + 1, bd);
     __syncthreads();
-    for (int z = zb; z < ze; ++z) {
-      const int base = z - zb;  // ring slot of plane z - 1 (mod 4)
-      for (int st = 0; st < TY * XT; st += 32) {
-        const int ry = st / XT, rx = st - ry * XT;  // row, x of the step
-        bf16x8 a[MT];
+    for (int zp = zb - 1; zp <= ze; ++zp) {
+      const int xs = (zp - zb + 1) % 3;
+      const int dbase = zp - zb + 3;  // dz slot of t is (dbase - t) & 3
+      const unsigned *xplane = xT + xs * PW;
 #pragma unroll
-        for (int m = 0; m < MT; ++m) {
-          const int co = m * 16 + r;
-          a[m] = co < COUT ? *reinterpret_cast<const bf16x8 *>(
-                                 dzT + co * (TY * XT) + st + 8 * g)
-                           : Zero8();
-          if (wv == 0) {
+      for (int j = 0; j < NSTEP / WS; ++j) {
+        const int st = (ws * (NSTEP / WS) + j) * 32;  // first position
+        const int ry = st / XT, rx = st - ry * XT;
+        bf16x8 a[MM];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) bsum[m] += (float)a[m][j];
+        for (int m = 0; m < MM; ++m) {
+          a[m] = at[m] < 3
+                     ? *reinterpret_cast<const bf16x8 *>(
+                           dzT + ((dbase - at[m]) & 3) * DPL + aco[m] * DROW +
+                           st + 8 * g)
+                     : Zero8();
+          if (wt == 0 && at[m] == 1) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) bsum[m] += (float)a[m][e];
           }
         }
 #pragma unroll
         for (int i = 0; i < TPW; ++i) {
-          if (wv + TY * i >= NTN) break;  // wave-uniform
+          if (wt + WT * i >= NTN) break;  // wave-uniform
           bf16x8 b = Zero8();
           if (lofs[i] >= 0) {
             const int e = lofs[i] + ry * CIN * XR + rx + 8 * g;
-            const unsigned *wsrc =
-                xT + ((base + lpl[i]) & 3) * PW + (e >> 1);
+            const unsigned *wsrc = xplane + (e >> 1);
             unsigned w[5];
 #pragma unroll
             for (int k = 0; k < 5; ++k) w[k] = wsrc[k];
@@ -495,42 +526,80 @@ __global__ __launch_bounds__(64 * TY) void WgradZKernel(
             b = __builtin_bit_cast(bf16x8, o);
           }
 #pragma unroll
-          for (int m = 0; m < MT; ++m)
+          for (int m = 0; m < MM; ++m)
             acc[i][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b,
                                                                 acc[i][m], 0,
                                                                 0, 0);
         }
       }
-      __syncthreads();  // every read of dzT and of plane z - 1 is done
-      putx((base + 3) & 3, bx);
-      putd(bd);
+      // x plane zp + 1 and dz plane zp + 2 into the slots no wave reads in
+      // this step (x: plane zp - 2, dz: plane zp - 2)
+      putx((zp - zb + 2) % 3, bx);
+      putd((zp - zb + 4) & 3, bd);
       __syncthreads();
-      fetchx(z + 3, bx);
-      fetchd(z + 2, bd);
+      fetchx(zp + 2, bx);
+      fetchd(zp + 3, bd);
     }
   }
+  // step groups 1 .. WS-1 hand their sums to group 0 through LDS
+  if constexpr (WS > 1) {
+    __syncthreads();  // the staged planes are no longer read
+    float *red = reinterpret_cast<float *>(lds);
+    constexpr int PER = TPW * MM * 4 + MM;  // floats per lane and wave
+    if (ws > 0) {
+      float *o = red + ((ws - 1) * WT + wt) * PER * 64;
 #pragma unroll
-  for (int i = 0; i < TPW; ++i) {
-    const int col = (wv + TY * i) * 16 + r;
-    if (wv + TY * i >= NTN || col >= K) continue;
+      for (int i = 0; i < TPW; ++i)
 #pragma unroll
-    for (int m = 0; m < MT; ++m)
+        for (int m = 0; m < MM; ++m)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int co = m * 16 + 4 * g + v;
-        if (co < COUT) part[(bid * COUT + co) * K + col] = acc[i][m][v];
+          for (int v = 0; v < 4; ++v)
+            o[((i * MM + m) * 4 + v) * 64 + lane] = acc[i][m][v];
+#pragma unroll
+      for (int m = 0; m < MM; ++m) o[(TPW * MM * 4 + m) * 64 + lane] = bsum[m];
+    }
+    __syncthreads();
+    if (ws == 0) {
+      for (int u = 1; u < WS; ++u) {
+        const float *o = red + ((u - 1) * WT + wt) * PER * 64;
+#pragma unroll
+        for (int i = 0; i < TPW; ++i)
+#pragma unroll
+          for (int m = 0; m < MM; ++m)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+              acc[i][m][v] += o[((i * MM + m) * 4 + v) * 64 + lane];
+#pragma unroll
+        for (int m = 0; m < MM; ++m)
+          bsum[m] += o[(TPW * MM * 4 + m) * 64 + lane];
       }
+    }
   }
-  if (wv == 0) {
-    // lanes r (channel 16 m + r) of the four groups g hold disjoint
-    // positions: fold the groups in order
+  if (ws == 0) {
 #pragma unroll
-    for (int m = 0; m < MT; ++m) {
-      float s = bsum[m];
-      s += __shfl_xor(s, 16, 64);
-      s += __shfl_xor(s, 32, 64);
-      const int co = m * 16 + r;
-      if (g == 0 && co < COUT) bpart[bid * COUT + co] = s;
+    for (int i = 0; i < TPW; ++i) {
+      const int col = (wt + WT * i) * 16 + r;
+      if (wt + WT * i >= NTN || col >= NC) continue;
+#pragma unroll
+      for (int m = 0; m < MM; ++m)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int q = 16 * m + 4 * g + v;
+          if (q >= 3 * COUT) continue;
+          const int t = q / COUT, co = q - t * COUT;
+          part[(bid * COUT + co) * K + 9 * t * CIN + col] = acc[i][m][v];
+        }
+    }
+    if (wt == 0) {
+      // lanes r of the four groups g hold disjoint positions of row q = 16 m
+      // + r: fold the groups in order
+#pragma unroll
+      for (int m = 0; m < MM; ++m) {
+        float sm = bsum[m];
+        sm += __shfl_xor(sm, 16, 64);
+        sm += __shfl_xor(sm, 32, 64);
+        if (g == 0 && at[m] == 1) bpart[bid * COUT + aco[m]] = sm;
+      }
     }
   }
 }
@@ -1243,6 +1312,11 @@ int WgradChunk(long P) {
 // 16 channels, else 32; 4 rows per block; z chunks as the convolution.
 // Returns the block count (= partials).
 constexpr int kWgradZRows = 4;
+// Wave groups over the column tiles (9 cin / 16 of them): one up to 5
+// tiles, two up to 14, else four (the rest of the 4 waves split the steps).
+constexpr int WgradZGroups(int cin) {
+  return (9 * cin + 15) / 16 <= 5 ? 1 : ((9 * cin + 15) / 16 <= 14 ? 2 : 4);
+}
 int WgradZXT(int cin, int W) { return (W % 64 == 0 && cin <= 16) ? 64 : 32; }
 long WgradZBlocks(int cin, int N, int D, int H, int W, int *tz, int *zchunk) {
   const int xt = WgradZXT(cin, W);
@@ -1285,13 +1359,14 @@ void WgradT(const u16 *x, const u16 *dz, float *part, float *bpart, int N,
     const dim3 grid(tiles, N * tz), block(64 * kWgradZRows);
     if (WgradZXT(CIN, W) == 64) {
       if constexpr (CIN <= 16)
-        hipLaunchKernelGGL((WgradZKernel<CIN, COUT, 64, kWgradZRows>), grid,
-                           block, 0, Stream(), x, dz, part, bpart, D, H, W,
-                           tz, zchunk);
+        hipLaunchKernelGGL(
+            (WgradZKernel<CIN, COUT, 64, kWgradZRows, WgradZGroups(CIN)>),
+            grid, block, 0, Stream(), x, dz, part, bpart, D, H, W, tz,
+            zchunk);
     } else {
-      hipLaunchKernelGGL((WgradZKernel<CIN, COUT, 32, kWgradZRows>), grid,
-                         block, 0, Stream(), x, dz, part, bpart, D, H, W, tz,
-                         zchunk);
+      hipLaunchKernelGGL(
+          (WgradZKernel<CIN, COUT, 32, kWgradZRows, WgradZGroups(CIN)>), grid,
+          block, 0, Stream(), x, dz, part, bpart, D, H, W, tz, zchunk);
     }
     Check();
     *nparts = (int)nb;
