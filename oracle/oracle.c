@@ -236,6 +236,33 @@ int oracle_himeno_run_omp(float *p0, float *p1, const float *const *coef,
   return threads;
 }
 
+/* ------------------------------------------ red-black 7-point relaxation */
+/* count iterations of physis_amd/programs/redblack7.c, each a red then a
+ * black colour sweep in place over [1, n-1) of every axis, in the
+ * REFERENCE target's loop order (reference_runtime_builder.cc:585-599,
+ * 617-646: z, y, then x from min + ((min & 1) ^ ((c + y + z) % 2)) in
+ * steps of 2, i.e. the points with x + y + z + c even).  coef: ce cw cn cs
+ * ct cb cc, summed in the kernel's operand order. */
+void oracle_redblack7_run(float *g, int nx, int ny, int nz, int count,
+                          const float *coef) {
+  const float ce = coef[0], cw = coef[1], cn = coef[2], cs = coef[3];
+  const float ct = coef[4], cb = coef[5], cc = coef[6];
+#define G(a, b, c) g[OFF(a, b, c, nx, ny)]
+  for (int it = 0; it < count; ++it)
+    for (int c = 0; c < 2; ++c)
+      for (int z = 1; z < nz - 1; ++z)
+        for (int y = 1; y < ny - 1; ++y)
+          for (int x = 1 + ((1 & 1) ^ ((c + y + z) % 2)); x < nx - 1;
+               x += 2) {
+            float v = cc * G(x, y, z) + cw * G(x - 1, y, z) +
+                      ce * G(x + 1, y, z) + cs * G(x, y + 1, z) +
+                      cn * G(x, y - 1, z) + cb * G(x, y, z - 1) +
+                      ct * G(x, y, z + 1);
+            G(x, y, z) = v;
+          }
+#undef G
+}
+
 /* ---------------------------------------------------- system-test kernels */
 #define FOR_DOMAIN(lo, hi)                  \
   for (int z = (lo); z < (hi); ++z)         \

@@ -97,6 +97,12 @@ int oracle_himeno_run_omp(float *p0, float *p1, const float *const *coef,
                           int nx, int ny, int nz, int nn, float omega,
                           int div, double *gosa);
 
+/* ---- red-black 7-point relaxation (physis_amd/programs/redblack7.c):
+ * count iterations of a red then a black in-place sweep over [1, n-1)^3
+ * in REFERENCE loop order; coef = ce cw cn cs ct cb cc. */
+void oracle_redblack7_run(float *g, int nx, int ny, int nz, int count,
+                          const float *coef);
+
 /* ---- system-test kernels (tests/system_tests/test_cases), N^3 float/int
  * grids, each one map application in -> out over the test's domain. */
 void oracle_test_7pt(const float *in, float *out, int n);           /* test_7-pt.c:12-20 */

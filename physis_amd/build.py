@@ -59,6 +59,7 @@ PHYSISC = os.path.join(PKG, "bin", "physisc-hip")
 # the reference's system tests) under tests/programs.
 PROGRAM_SOURCES = [os.path.join(PROGRAMS, "diffusion7.c"),
                    os.path.join(PROGRAMS, "himeno.c"),
+                   os.path.join(PROGRAMS, "redblack7.c"),
                    os.path.join(ROOT, "tests", "programs", "systests.c")]
 
 

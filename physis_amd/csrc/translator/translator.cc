@@ -674,6 +674,7 @@ void Translator::AnalyzeFusion(KernelInfo &k) {
   if (!cfg_.two_sweep) return no("HIP_TWO_SWEEP = false");
   if (!k.fast || k.ndims != 3) return no("not a 2.5D run kernel");
   if (k.index_rebound) return no("an index parameter is rebound");
+  if (k.used_rb) return no("used in red-black maps");
   int in = -1, out = -1, direct = 0;
   for (int i = 0; i < (int)k.grids.size(); ++i) {
     const GridParam &g = k.grids[i];
@@ -742,8 +743,6 @@ void Translator::Decide(KernelInfo &k) {
       k.slow_reason = "grid " + g.name + " holds user-defined points";
     }
   if (k.used_rb) {
-    k.fast = false;
-    k.slow_reason = "red-black map";
     // A colour sweep updates its grid in place.  REF visits the points of
     // a colour in loop order, so it equals a parallel sweep only when no
     // point reads another point of its own colour: every regular get of an
@@ -760,9 +759,19 @@ void Translator::Decide(KernelInfo &k) {
     k.rb_note = exact ? "in-place colour sweep reads only the other colour"
                       : "in-place colour sweep reads its own colour: results "
                         "depend on the order of updates";
-    if (!exact)
+    if (!exact) {
       std::fprintf(stderr, "physisc-hip: warning: %s: %s\n", k.name.c_str(),
                    k.rb_note.c_str());
+      // the generic kernel's one thread per column keeps the REF order
+      // along z at least
+      k.fast = false;
+      k.slow_reason = "red-black map reading its own colour";
+    }
+    // Otherwise the 2.5D kernel runs a colour as a masked in-place sweep:
+    // every value it reads of the emitted grid is of the other colour
+    // (never written in this sweep) or the point's own centre (written only
+    // by its own lane, after the read), so staged planes, halo rows of
+    // neighbouring tiles and neighbouring z chunks are read race-free.
   }
   for (auto &g : k.grids)
     if (g.dims != 3 && (g.read || g.emitted)) {
@@ -1234,6 +1243,7 @@ std::string Translator::EmitFastRunKernel(const KernelInfo &k) {
   for (auto &s : k.scalars) params.push_back(s.type + " " + s.name);
   params.push_back("physis_hip::RunShape sh");
   params.push_back(RT + " *__restrict__ __red");
+  if (k.used_rb) params.push_back("int __rb");
   o << "// Run kernel: VX points per lane, BY waves per block (the first and\n"
        "// last " << k.hy << " load halo rows), staged planes PD steps ahead,\n"
        "// centre-only grids one step ahead.  RED: reduction operator of a\n"
@@ -1386,8 +1396,10 @@ std::string Translator::EmitFastRunKernel(const KernelInfo &k) {
   std::ostringstream b;
   b << "#pragma unroll\n          for (int i = 0; i < VX; ++i) {\n"
        "            const int x = xs + i;\n"
-       "            const bool __in = x >= dom.local_min[0] && x < dom.local_max[0];\n"
-       "            {\n";
+       "            const bool __in = x >= dom.local_min[0] && x < dom.local_max[0]"
+    << (k.used_rb ? " &&\n                (__rb < 0 || ((x + yrow + z + __rb) & 1) == 0)"
+                  : "")
+    << ";\n            {\n";
   std::ostringstream &ob = b;
   {
   std::ostringstream &o = ob;
@@ -1886,24 +1898,31 @@ std::string Translator::EmitLauncher(const KernelInfo &k) {
       << "  return physis_hip::PlanRun(s.dom, 64 * VX, " << by << " - 2 * "
       << k.hy << ", per_cu, physis_hip::WindowChunkLimit({" << Join(wgrids, ", ")
       << "}, " << margin << "));\n}\n\n";
+    // red-black maps: one launch per colour, red first
+    const std::string RBA = k.used_rb ? ", __c" : "";
     o << "template <int VX>\nstatic int __PSLaunch_" << n << "(const struct __PSStencil_"
       << n << " &s, hipStream_t stream, void *red) {\n"
       << "  const physis_hip::RunShape sh = __PSShape_" << n << "<VX>(s);\n"
       << "  if (sh.blocks == 0) return 0;\n"
       << "  const dim3 grid(sh.blocks), block(64 * " << by << ");\n";
+    if (k.used_rb)
+      o << "  for (int __c = s.__rb == 2 ? 0 : s.__rb; "
+           "__c <= (s.__rb == 2 ? 1 : s.__rb); ++__c) {\n";
+    else
+      o << "  {\n";
     if (red && k.used_reduce) {
       o << "  switch (red ? s.__reduce_op : -1) {\n";
       for (const char *op : {"PS_MAX", "PS_MIN", "PS_SUM", "PS_PROD"})
         o << "    case " << op << ":\n      hipLaunchKernelGGL((__PSStencilRun_" << n
           << "<VX, " << by << ", " << op << ">), grid, block, 0, stream, " << A
-          << ", sh, (" << RT << " *)red);\n      break;\n";
+          << ", sh, (" << RT << " *)red" << RBA << ");\n      break;\n";
       o << "    default:\n";
     } else {
       o << "  (void)red;\n  {\n";
     }
     o << "      hipLaunchKernelGGL((__PSStencilRun_" << n << "<VX, " << by
       << ", -1>), grid, block, 0, stream, " << A << ", sh, (" << RT
-      << " *)NULL);\n  }\n  return sh.blocks;\n}\n\n";
+      << " *)NULL" << RBA << ");\n  }\n  }\n  return sh.blocks;\n}\n\n";
     // vector width: every 3-D grid's x extent a multiple of 4
     std::vector<std::string> conds;
     for (auto &g : k.grids)

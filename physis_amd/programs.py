@@ -43,6 +43,21 @@ class Diffusion7:
                                          *[float(c) for c in coef])
 
 
+class RedBlack7:
+    """physis_amd/programs/redblack7.c: red-black Gauss-Seidel relaxation of
+    the 7-point diffusion operator, in place over the grid's interior."""
+
+    def __init__(self):
+        L = rt.load_program("redblack7")
+        L.redblack7_run.argtypes = [rt.PSGridPtr, C.c_int] + [_F] * 7
+        self.lib = L
+
+    def run(self, g, count: int, coef) -> None:
+        """count iterations (red then black sweep) of grid g.
+        coef = (ce, cw, cn, cs, ct, cb, cc)."""
+        self.lib.redblack7_run(g.handle, count, *[float(c) for c in coef])
+
+
 class Himeno:
     """physis_amd/programs/himeno.c: 19-point Himeno Jacobi sweeps."""
 

@@ -47,6 +47,11 @@ def run_system_test(name):
         return ("%f\n" % L.oracle_reduce_float(2, g, g.size)).encode()
     n = 32
     g1, g2 = iota(n ** 3), iota(n ** 3)
+    if name == "test_redblack":
+        # the unweighted in-place sum: integer values below 2^24, so the
+        # sum is exact in any operand order
+        L.oracle_redblack7_run(g1, n, n, n, 1, np.ones(7, np.float32))
+        return dump_f(g1)
     if name == "test_7-pt-multi-iterations":
         for _ in range(5):
             L.oracle_test_7pt(g1, g2, n)
@@ -72,7 +77,7 @@ ORACLE_SYSTEM_TESTS = [
     "test_7-pt", "test_7-pt-periodic", "test_7-pt-neumann-cond",
     "test_7-pt-multi-iterations", "test_asymmetric", "test_asymmetric-periodic",
     "test_27-pt", "test_27-pt-periodic", "test_27-pt-reduction",
-    "test_reduction-3d-sum"]
+    "test_reduction-3d-sum", "test_redblack"]
 
 
 @pytest.mark.parametrize("name", ORACLE_SYSTEM_TESTS)
@@ -310,3 +315,34 @@ def test_reduction_semantics():
     g = iota(16 ** 3)
     assert L.oracle_reduce_float(2, g, g.size) == 8386560.0
     assert L.oracle_reduce_float_f64(2, g, g.size) == 8386560.0
+
+
+def test_redblack_oracle_is_order_free():
+    """Every neighbour of a point is of the other colour, so a colour sweep
+    in any order gives the REFERENCE loop order's bits: a vectorised sweep
+    (each colour computed from the array as the previous colour left it)
+    equals the oracle's in-place loops with the diffusion weights."""
+    nx, ny, nz = 11, 8, 9
+    rng = np.random.default_rng(3)
+    g = rng.random(nx * ny * nz).astype(np.float32)
+    coef = np.array([0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.4], np.float32)
+    o = g.copy()
+    lib().oracle_redblack7_run(o, nx, ny, nz, 2, coef)
+    a = g.reshape(nz, ny, nx).copy()
+    ce, cw, cn, cs, ct, cb, cc = coef
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx),
+                          indexing="ij")
+    inner = ((x > 0) & (x < nx - 1) & (y > 0) & (y < ny - 1) & (z > 0) &
+             (z < nz - 1))
+    for _ in range(2):
+        for c in range(2):
+            m = inner & ((x + y + z + c) % 2 == 0)
+            v = np.zeros_like(a)
+            s = (slice(1, -1),) * 3
+            v[s] = (cc * a[s] + cw * a[1:-1, 1:-1, :-2] +
+                    ce * a[1:-1, 1:-1, 2:] + cs * a[1:-1, 2:, 1:-1] +
+                    cn * a[1:-1, :-2, 1:-1] + cb * a[:-2, 1:-1, 1:-1] +
+                    ct * a[2:, 1:-1, 1:-1])
+            a[m] = v[m]
+    assert np.array_equal(a.reshape(-1), o)
+    assert not np.array_equal(o, g)
