@@ -249,6 +249,28 @@ __global__ __launch_bounds__(64 * TY) void ConvZKernel(
       *reinterpret_cast<uint4 *>(wlds + v * 8) = u;
     }
   }
+  // With the weights in registers and at most 14 K steps the K loop is
+  // unrolled and the B operand's LDS offset of every K step (u16 elements
+  // within a staged plane, relative to the lane's row and x) is decoded
+  // once, with the tap's plane (0..2) in bits 24 and up; -1 past K.
+  // (Measured at level 1: backward data 8 -> 24 392 -> 279 us, forward
+  // 8 -> 8 198 -> 187 us; the 21-step 24 -> 8 forward went 617 -> 729 us
+  // and keeps the per-step decode.)
+  constexpr bool kDecode = kRegW && KB <= 14;
+  int boff[kDecode ? KB : 1];
+  if constexpr (kDecode) {
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      const int k0 = kb * 32 + 8 * g;
+      if (k0 < K) {
+        const int tap = k0 / CIN, ci0 = k0 - tap * CIN;
+        boff[kb] = ((((tap / 3) % 3) * XS + tap % 3) * CIN + ci0) |
+                   ((tap / 9) << 24);
+      } else {
+        boff[kb] = -1;
+      }
+    }
+  }
   // bias of the four channels co = 16 t + 4 g + v this lane receives
   float bb[MT][4];
 #pragma unroll
@@ -269,27 +291,47 @@ __global__ __launch_bounds__(64 * TY) void ConvZKernel(
 #pragma unroll
         for (int t = 0; t < MT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
         const int xa = xs0 + r;
-#pragma unroll 4
-        for (int kb = 0; kb < KB; ++kb) {
-          const int k0 = kb * 32 + 8 * g;
-          bf16x8 b = Zero8();
-          if (k0 < K) {
-            const int tap = k0 / CIN, ci0 = k0 - tap * CIN;
-            const int slot = (base + tap / 9) & 3;
-            b = *reinterpret_cast<const bf16x8 *>(
-                ring + slot * PLANE +
-                ((wv + (tap / 3) % 3) * XS + xa + tap % 3) * CIN + ci0);
-          }
+        if constexpr (kDecode) {
+          const u16 *rb = ring + (wv * XS + xa) * CIN;
+          const int so[3] = {(base & 3) * PLANE, ((base + 1) & 3) * PLANE,
+                             ((base + 2) & 3) * PLANE};
 #pragma unroll
-          for (int t = 0; t < MT; ++t) {
-            bf16x8 a;
-            if constexpr (kRegW)
-              a = aw[kb][t];
-            else
-              a = *reinterpret_cast<const bf16x8 *>(
-                  wlds + (t * 16 + r) * KP + kb * 32 + 8 * g);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0,
-                                                             0, 0);
+          for (int kb = 0; kb < KB; ++kb) {
+            bf16x8 b = Zero8();
+            if (boff[kb] >= 0) {
+              const int pz = boff[kb] >> 24;
+              const int sp = pz == 0 ? so[0] : (pz == 1 ? so[1] : so[2]);
+              b = *reinterpret_cast<const bf16x8 *>(rb + sp +
+                                                    (boff[kb] & 0xffffff));
+            }
+#pragma unroll
+            for (int t = 0; t < MT; ++t)
+              acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                  aw[kb][t], b, acc[t], 0, 0, 0);
+          }
+        } else {
+#pragma unroll 4
+          for (int kb = 0; kb < KB; ++kb) {
+            const int k0 = kb * 32 + 8 * g;
+            bf16x8 b = Zero8();
+            if (k0 < K) {
+              const int tap = k0 / CIN, ci0 = k0 - tap * CIN;
+              const int slot = (base + tap / 9) & 3;
+              b = *reinterpret_cast<const bf16x8 *>(
+                  ring + slot * PLANE +
+                  ((wv + (tap / 3) % 3) * XS + xa + tap % 3) * CIN + ci0);
+            }
+#pragma unroll
+            for (int t = 0; t < MT; ++t) {
+              bf16x8 a;
+              if constexpr (kRegW)
+                a = aw[kb][t];
+              else
+                a = *reinterpret_cast<const bf16x8 *>(
+                    wlds + (t * 16 + r) * KP + kb * 32 + 8 * g);
+              acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t],
+                                                               0, 0, 0);
+            }
           }
         }
         const int xo = x0 + xs0 + r;
@@ -312,6 +354,142 @@ __global__ __launch_bounds__(64 * TY) void ConvZKernel(
             *reinterpret_cast<uint2 *>(y + (row + xo) * COUT + co) =
                 make_uint2(w2[0], w2[1]);
           }
+        }
+      }
+    }
+    put((base + 3) & 3, buf);
+    __syncthreads();
+    fetch(z + 3, buf);
+  }
+}
+
+// The z-marching convolution for 8 output channels, two output rows per
+// wave.  With COUT = 8 half of the 16 A rows of the tap-major kernel are
+// zero; here row q = 8 h + co of the A operand is output channel co of
+// output row y + h (h = 0, 1), and K runs over the input rows y - 1 .. y + 2
+// of both: k = ((dz 4 + ry) 3 + dx) CIN + ci, A[q][k] = W[co][9 dz + 3 (ry -
+// h) + dx][ci] when 0 <= ry - h <= 2, else 0.  One B operand (8 channels of
+// one staged input position) then serves both rows: 36 CIN / 32 K steps per
+// 16 x positions of two rows instead of 2 x 27 CIN / 32 (CIN = 24: 27
+// instead of 42 MFMAs and B reads).  Block (x tile, 2 TY rows, problem, z
+// chunk); the ring and the staging are those of ConvZKernel.
+template <int CIN, bool RELU, int XT, int TY>
+__global__ __launch_bounds__(64 * TY) void ConvZPairKernel(
+    const u16 *__restrict__ x, const u16 *__restrict__ wt,
+    const float *__restrict__ bias, u16 *__restrict__ y, int D, int H, int W,
+    int tz, int zchunk) {
+  constexpr int COUT = 8;
+  constexpr int KW = 27 * CIN, KPW = ((KW + 31) / 32) * 32;  // wt layout
+  constexpr int K = 36 * CIN, KB = (K + 31) / 32;
+  constexpr int ROWS = 2 * TY;
+  constexpr int XS = XT + 2, RS = ROWS + 2, NV = CIN / 8;
+  constexpr int PLANE = RS * XS * CIN;  // u16 elements of a staged plane
+  constexpr int SV = RS * XS * NV;      // 16-byte vectors of a staged plane
+  constexpr int NTH = 64 * TY;
+  constexpr int VPT = (SV + NTH - 1) / NTH;
+  __shared__ __attribute__((aligned(16))) u16 ring[4 * PLANE];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const int tiles_x = (W + XT - 1) / XT;
+  const int x0 = (blockIdx.x % tiles_x) * XT;
+  const int y0 = (blockIdx.x / tiles_x) * ROWS;
+  const int n = blockIdx.y / tz;
+  const int zb = (blockIdx.y - n * tz) * zchunk;
+  const int ze = min(zb + zchunk, D);
+  if (zb >= ze) return;  // uniform over the block
+  auto fetch = [&](int zz, uint4 (&buf)[VPT]) {
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) {
+      const int v = threadIdx.x + i * NTH;
+      const int c8 = v % NV, t = v / NV;
+      const int xp = t % XS, pr = t / XS;
+      const int yy = y0 - 1 + pr, xx = x0 - 1 + xp;
+      uint4 u = make_uint4(0, 0, 0, 0);
+      if (v < SV && zz >= 0 && zz < D && yy >= 0 && yy < H && xx >= 0 &&
+          xx < W)
+        u = *reinterpret_cast<const uint4 *>(
+            x + ((((long)n * D + zz) * H + yy) * W + xx) * CIN + c8 * 8);
+      buf[i] = u;
+    }
+  };
+  auto put = [&](int slot, const uint4 (&buf)[VPT]) {
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) {
+      const int v = threadIdx.x + i * NTH;
+      if (v < SV)
+        *reinterpret_cast<uint4 *>(ring + slot * PLANE + v * 8) = buf[i];
+    }
+  };
+  uint4 buf[VPT];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    fetch(zb - 1 + k, buf);
+    put(k, buf);
+  }
+  fetch(zb + 2, buf);
+  // A fragments (registers) and the B operand's offsets of every K step:
+  // u16 elements within a staged plane relative to the wave's first row and
+  // the lane's x, with the input plane (0..2) in bits 24 and up; -1 past K
+  const int co = r & 7, h = r >> 3;
+  bf16x8 aw[KB];
+  int boff[KB];
+#pragma unroll
+  for (int kb = 0; kb < KB; ++kb) {
+    const int k0 = kb * 32 + 8 * g;
+    aw[kb] = Zero8();
+    boff[kb] = -1;
+    if (k0 < K) {
+      const int cell = k0 / CIN, ci0 = k0 - cell * CIN;
+      const int dz = cell / 12, ry = (cell / 3) % 4, dx = cell % 3;
+      const int dy = ry - h;
+      if (dy >= 0 && dy <= 2)
+        aw[kb] = Load8(wt + (long)co * KPW + (9 * dz + 3 * dy + dx) * CIN + ci0);
+      boff[kb] = ((ry * XS + dx) * CIN + ci0) | (dz << 24);
+    }
+  }
+  // bias of the four outputs (row y + (g >> 1), channels 4 (g & 1) + v)
+  float bb[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) bb[v] = bias ? bias[4 * (g & 1) + v] : 0.0f;
+  __syncthreads();
+  const int yy0 = y0 + 2 * wv;  // the wave's first output row
+  const int yo = yy0 + (g >> 1);  // the row of this lane's outputs
+  for (int z = zb; z < ze; ++z) {
+    const int base = z - zb;  // ring slot of plane z - 1 (mod 4)
+    if (yy0 < H) {
+      const long row = (((long)n * D + z) * H + yo) * W;
+      const int so[3] = {(base & 3) * PLANE, ((base + 1) & 3) * PLANE,
+                         ((base + 2) & 3) * PLANE};
+      for (int xs0 = 0; xs0 < XT && x0 + xs0 < W; xs0 += 16) {
+        f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+        const u16 *rb = ring + (2 * wv * XS + xs0 + r) * CIN;
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) {
+          bf16x8 b = Zero8();
+          if (boff[kb] >= 0) {
+            const int pz = boff[kb] >> 24;
+            const int sp = pz == 0 ? so[0] : (pz == 1 ? so[1] : so[2]);
+            b = *reinterpret_cast<const bf16x8 *>(rb + sp +
+                                                  (boff[kb] & 0xffffff));
+          }
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw[kb], b, acc, 0, 0,
+                                                        0);
+        }
+        const int xo = x0 + xs0 + r;
+        if (xo < W && yo < H) {
+          unsigned w2[2];
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            float v0 = acc[2 * q] + bb[2 * q];
+            float v1 = acc[2 * q + 1] + bb[2 * q + 1];
+            if (RELU) {
+              v0 = v0 > 0.0f ? v0 : 0.0f;
+              v1 = v1 > 0.0f ? v1 : 0.0f;
+            }
+            w2[q] = (unsigned)F2Bf(v0) | ((unsigned)F2Bf(v1) << 16);
+          }
+          *reinterpret_cast<uint2 *>(y + (row + xo) * COUT + 4 * (g & 1)) =
+              make_uint2(w2[0], w2[1]);
         }
       }
     }
@@ -1280,12 +1458,38 @@ void ConvZLaunch(const u16 *x, const u16 *wt, const float *b, bool relu,
   Check();
 }
 
+template <int CIN, int XT, int TY>
+void ConvZPairLaunch(const u16 *x, const u16 *wt, const float *b, bool relu,
+                     u16 *y, int N, int D, int H, int W) {
+  const int tiles = ((W + XT - 1) / XT) * ((H + 2 * TY - 1) / (2 * TY));
+  int tz, zchunk;
+  ZChunks(tiles, N, D, kConvZBlocksPerCU, &tz, &zchunk);
+  const dim3 grid(tiles, N * tz), block(64 * TY);
+  if (relu)
+    hipLaunchKernelGGL((ConvZPairKernel<CIN, true, XT, TY>), grid, block, 0,
+                       Stream(), x, wt, b, y, D, H, W, tz, zchunk);
+  else
+    hipLaunchKernelGGL((ConvZPairKernel<CIN, false, XT, TY>), grid, block, 0,
+                       Stream(), x, wt, b, y, D, H, W, tz, zchunk);
+  Check();
+}
+
 // z-marching convolution: x tile of 64 positions below 24 input channels,
 // else 32, 16 for rows of at most 16; 8 rows per block below 24 input
 // channels, else 4 (the ring of 4 staged planes then stays within LDS).
 template <int CIN, int COUT>
 void ConvT(const u16 *x, const u16 *wt, const float *b, bool relu, u16 *y,
            int N, int D, int H, int W) {
+  if constexpr (COUT == 8) {
+    // two output rows per wave: 8 rows per block of 4 waves
+    if constexpr (CIN < 24) {
+      if (W > 32)
+        return ConvZPairLaunch<CIN, 64, 4>(x, wt, b, relu, y, N, D, H, W);
+    }
+    if (W > 16)
+      return ConvZPairLaunch<CIN, 32, 4>(x, wt, b, relu, y, N, D, H, W);
+    return ConvZPairLaunch<CIN, 16, 4>(x, wt, b, relu, y, N, D, H, W);
+  }
   constexpr int TY = CIN < 24 ? 8 : 4;
   if constexpr (CIN < 24) {
     if (W > 32)

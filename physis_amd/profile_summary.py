@@ -197,11 +197,15 @@ def unet_md(kt_dir: str, out_md: str, command: str) -> None:
         for r in csv.DictReader(open(f)):
             m = re.search(r"ConvZ?Kernel<(\d+), (\d+), (true|false)",
                           r["Name"])
+            mp = re.search(r"ConvZPairKernel<(\d+), (true|false)", r["Name"])
             w = re.search(r"Wgrad(?:Lds|Z)?Kernel<(\d+), (\d+)",
                           r["Name"])
             if m:
                 cin, cout = int(m.group(1)), int(m.group(2))
                 what = "forward" if m.group(3) == "true" else "backward data"
+            elif mp:  # 8 output channels, two rows per wave
+                cin, cout = int(mp.group(1)), 8
+                what = "forward" if mp.group(2) == "true" else "backward data"
             elif w:
                 cin, cout = int(w.group(1)), int(w.group(2))
                 what = "backward filter"
@@ -245,9 +249,9 @@ STENCIL_FRAGS = [
     ("512 MiB copy", "copyBuffer")]
 # the level-1 kernels of physis_amd.profile_unet (batch 8 of 128^3)
 UNET_FRAGS = [
-    ("forward 8->8", "ConvZKernel<8, 8, true"),
-    ("forward 24->8", "ConvZKernel<24, 8, true"),
-    ("backward data 8->24", "ConvZKernel<24, 8, false"),
+    ("forward 8->8", "ConvZPairKernel<8, true"),
+    ("forward 24->8", "ConvZPairKernel<24, true"),
+    ("backward data 8->24", "ConvZKernel<8, 24, false"),
     ("backward filter 8->8", "WgradZKernel<8, 8,"),
     ("backward filter 24->8", "WgradZKernel<24, 8,")]
 
