@@ -1424,12 +1424,12 @@ void ConvLaunch(const u16 *x, const u16 *wt, const float *b, bool relu,
   Check();
 }
 
-// Blocks per compute unit that the z-marching launches aim for (the
-// filter gradient's inner loop waits on LDS between barriers and wants more
-// resident waves than the convolution: 8 -> 8 at level 1 takes 480 us with
-// 4 per CU against 693 with 2; the convolutions are fastest with 2).
+// Blocks per compute unit that the z-marching launches aim for (measured
+// at level 1 of batch 8 x 128^3: the plane-marching filter gradient is
+// fastest with 2, e.g. 8 -> 8 213 us against 223 with 4 and 237 with 8,
+// and so are the convolutions).
 constexpr int kConvZBlocksPerCU = 2;
-constexpr int kWgradZBlocksPerCU = 4;
+constexpr int kWgradZBlocksPerCU = 2;
 
 // z chunks of a z-marching launch: enough blocks for per_cu per compute
 // unit over `tiles` xy tiles of N problems of D planes.
@@ -1513,7 +1513,8 @@ int WgradChunk(long P) {
 
 // Launch shape of the z-marching filter gradient (rows of a multiple of 32
 // positions): x tile 64 when the row allows it and the input has at most
-// 16 channels, else 32; 4 rows per block; z chunks as the convolution.
+// 24 channels (24 -> 8 at level 1: 479 us against 526 with 32), else 32;
+// 4 rows per block; z chunks as the convolution.
 // Returns the block count (= partials).
 constexpr int kWgradZRows = 4;
 // Wave groups over the column tiles (9 cin / 16 of them): one up to 5
@@ -1521,7 +1522,7 @@ constexpr int kWgradZRows = 4;
 constexpr int WgradZGroups(int cin) {
   return (9 * cin + 15) / 16 <= 5 ? 1 : ((9 * cin + 15) / 16 <= 14 ? 2 : 4);
 }
-int WgradZXT(int cin, int W) { return (W % 64 == 0 && cin <= 16) ? 64 : 32; }
+int WgradZXT(int cin, int W) { return (W % 64 == 0 && cin <= 24) ? 64 : 32; }
 long WgradZBlocks(int cin, int N, int D, int H, int W, int *tz, int *zchunk) {
   const int xt = WgradZXT(cin, W);
   const long tiles = (long)((W + xt - 1) / xt) *
@@ -1562,7 +1563,7 @@ void WgradT(const u16 *x, const u16 *dz, float *part, float *bpart, int N,
     const int tiles = (int)(nb / ((long)N * tz));
     const dim3 grid(tiles, N * tz), block(64 * kWgradZRows);
     if (WgradZXT(CIN, W) == 64) {
-      if constexpr (CIN <= 16)
+      if constexpr (CIN <= 24)
         hipLaunchKernelGGL(
             (WgradZKernel<CIN, COUT, 64, kWgradZRows, WgradZGroups(CIN)>),
             grid, block, 0, Stream(), x, dz, part, bpart, D, H, W, tz,
