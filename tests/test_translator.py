@@ -89,13 +89,19 @@ void run(PSGrid3DFloat g) {
 """
     rc, code, err = translate(None, src)
     assert rc == 0, err
-    assert "if (__rb >= 0 && ((x + y + z + __rb) & 1)) continue;" in code
+    # every get is of the other colour: the 2.5D kernel, the colour in the
+    # store mask (the points with x + y + z + rb even)
+    assert "(__rb < 0 || ((x + yrow + z + __rb) & 1) == 0)" in code
+    assert "__PSStencilRunGeneric_k" not in code
     assert "__PSStencilMapRedBlack_k(d, g)" in code
     assert "__PSStencilMapBlack_k(d, g)" in code
     assert "reads only the other colour" in code and not err
-    # a get of the point's own colour is flagged: REF order matters there
+    assert "no (used in red-black maps)" in code
+    # a get of the point's own colour is flagged and keeps the generic
+    # kernel, which skips the other colour's points
     rc, code, err = translate(None, src.replace("x+1, y, z", "x+1, y+1, z"))
     assert rc == 0 and "reads its own colour" in err
+    assert "if (__rb >= 0 && ((x + y + z + __rb) & 1)) continue;" in code
 
 
 def test_missing_kernel_is_an_error():
