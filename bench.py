@@ -425,7 +425,7 @@ def main() -> int:
     from physis_amd import runtime as ps
     from physis_amd.configs import (DIFFUSION_COEF, diffusion_input,
                                     himeno_inputs)
-    from physis_amd.programs import Diffusion7, Himeno, RedBlack7
+    from physis_amd.programs import Diffusion7, Himeno, RedBlack7, Udt7
 
     ps.PSInit()
     hip = Hip(ps.hip_stream())
@@ -525,6 +525,22 @@ def main() -> int:
     rb_pts = (N7 - 2) ** 3
     ps.PSGridFree(g1)
     ps.PSGridFree(g2)
+
+    # The same stencil over a grid of user-defined points (members p, q):
+    # q <- 7-point(p) over the interior, on the generic run kernel.
+    udt = Udt7()
+    pq = np.zeros((N7 ** 3, 2), np.float32)
+    pq[:, 0] = a
+    ug = udt.new((N7, N7, N7), pq)
+    del pq
+    udt.run(ug, 2, DIFFUSION_COEF)
+    udt_sweeps = 20
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0)
+    udt.run(ug, udt_sweeps, DIFFUSION_COEF)
+    hip.record(e1)
+    udt_ms = hip.ms(e0, e1) / udt_sweeps
+    udt.free(ug)
 
     # 19-point Himeno, 512x256x256, 400 sweeps with per-sweep gosa.
     log("generating Himeno 512x256x256 inputs")
@@ -626,6 +642,14 @@ def main() -> int:
             "note": "in-place red then black sweep over the 510^3 interior "
                     "of 512^3 per iteration; hbm_frac counts each colour "
                     "sweep's minimum bytes (read the grid, write half)"},
+        "user_type_7pt": {
+            "value": round(rb_pts / (udt_ms * 1e-3) / 1e9, 2),
+            "unit": "Gpts/s",
+            "ms_per_sweep": round(udt_ms, 5),
+            "hbm_frac": round(8 * N7 ** 3 / udt_ms / 1e6 / HBM_PEAK_GBS, 4),
+            "note": "points (p, q) stored member by member; q <- 7-point(p) "
+                    "over the 510^3 interior of 512^3 (generic run kernel); "
+                    "hbm_frac counts 8 B/pt (read p, write q)"},
         "himeno_19pt": {
             "grid": list(SHAPE19), "sweeps": 400,
             "value": round(n19 * 400 / t19 / 1e9, 2), "unit": "Gpts/s",

@@ -263,6 +263,26 @@ void oracle_redblack7_run(float *g, int nx, int ny, int nz, int count,
 #undef G
 }
 
+/* ------------------------------------ 7-point stencil on user-defined points */
+/* One sweep of physis_amd/programs/udt7.c over [1, n-1) of every axis:
+ * q[c] = cc p[c] + cw p[x-1] + ce p[x+1] + cs p[y+1] + cn p[y-1] + cb p[z-1]
+ * + ct p[z+1], left to right (points outside keep q).  p, q: the two
+ * members of the points, each a dense array (the runtime's layout). */
+void oracle_udt7_sweep(const float *p, float *q, int nx, int ny, int nz,
+                       const float *coef) {
+  const float ce = coef[0], cw = coef[1], cn = coef[2], cs = coef[3];
+  const float ct = coef[4], cb = coef[5], cc = coef[6];
+#define P(a, b, c) p[OFF(a, b, c, nx, ny)]
+  for (int z = 1; z < nz - 1; ++z)
+    for (int y = 1; y < ny - 1; ++y)
+      for (int x = 1; x < nx - 1; ++x)
+        q[OFF(x, y, z, nx, ny)] =
+            cc * P(x, y, z) + cw * P(x - 1, y, z) + ce * P(x + 1, y, z) +
+            cs * P(x, y + 1, z) + cn * P(x, y - 1, z) + cb * P(x, y, z - 1) +
+            ct * P(x, y, z + 1);
+#undef P
+}
+
 /* ---------------------------------------------------- system-test kernels */
 #define FOR_DOMAIN(lo, hi)                  \
   for (int z = (lo); z < (hi); ++z)         \

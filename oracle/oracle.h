@@ -103,6 +103,12 @@ int oracle_himeno_run_omp(float *p0, float *p1, const float *const *coef,
 void oracle_redblack7_run(float *g, int nx, int ny, int nz, int count,
                           const float *coef);
 
+/* ---- 7-point stencil on user-defined points (physis_amd/programs/udt7.c):
+ * one sweep q <- weighted 7-point sum of p over [1, n-1)^3; coef = ce cw cn
+ * cs ct cb cc. */
+void oracle_udt7_sweep(const float *p, float *q, int nx, int ny, int nz,
+                       const float *coef);
+
 /* ---- system-test kernels (tests/system_tests/test_cases), N^3 float/int
  * grids, each one map application in -> out over the test's domain. */
 void oracle_test_7pt(const float *in, float *out, int n);           /* test_7-pt.c:12-20 */

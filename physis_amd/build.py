@@ -60,6 +60,7 @@ PHYSISC = os.path.join(PKG, "bin", "physisc-hip")
 PROGRAM_SOURCES = [os.path.join(PROGRAMS, "diffusion7.c"),
                    os.path.join(PROGRAMS, "himeno.c"),
                    os.path.join(PROGRAMS, "redblack7.c"),
+                   os.path.join(PROGRAMS, "udt7.c"),
                    os.path.join(ROOT, "tests", "programs", "systests.c")]
 
 

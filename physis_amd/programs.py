@@ -58,6 +58,40 @@ class RedBlack7:
         self.lib.redblack7_run(g.handle, count, *[float(c) for c in coef])
 
 
+class Udt7:
+    """physis_amd/programs/udt7.c: a 7-point stencil over a grid of points
+    with members (p, q): q <- weighted 7-point sum of p over the interior."""
+
+    def __init__(self):
+        L = rt.load_program("udt7")
+        L.udt7_new.argtypes = [C.c_int] * 3
+        L.udt7_new.restype = C.c_void_p
+        L.udt7_copyin.argtypes = [C.c_void_p, C.c_void_p]
+        L.udt7_copyout.argtypes = [C.c_void_p, C.c_void_p]
+        L.udt7_free.argtypes = [C.c_void_p]
+        L.udt7_run.argtypes = [C.c_void_p, C.c_int] + [_F] * 7
+        self.lib = L
+
+    def new(self, shape, pq: np.ndarray):
+        """A grid of shape (nx, ny, nz) holding the points pq [n][2]."""
+        g = self.lib.udt7_new(*shape)
+        if not g:
+            raise RuntimeError("udt7_new failed")
+        self.lib.udt7_copyin(g, np.ascontiguousarray(pq, np.float32).ctypes.data)
+        return g
+
+    def copyout(self, g, n: int) -> np.ndarray:
+        out = np.empty((n, 2), np.float32)
+        self.lib.udt7_copyout(g, out.ctypes.data)
+        return out
+
+    def free(self, g) -> None:
+        self.lib.udt7_free(g)
+
+    def run(self, g, count: int, coef) -> None:
+        self.lib.udt7_run(g, count, *[float(c) for c in coef])
+
+
 class Himeno:
     """physis_amd/programs/himeno.c: 19-point Himeno Jacobi sweeps."""
 

@@ -71,6 +71,8 @@ def lib() -> C.CDLL:
         L.oracle_test_27pt.argtypes = [FP, FP, C.c_int, C.c_int]
         L.oracle_redblack7_run.argtypes = [FP, C.c_int, C.c_int, C.c_int,
                                            C.c_int, FP]
+        L.oracle_udt7_sweep.argtypes = [FP, FP, C.c_int, C.c_int, C.c_int,
+                                        FP]
         L.oracle_test_27pt_int.argtypes = [IP, IP, C.c_int]
         L.oracle_diffusion_coefs.argtypes = [C.c_int, C.c_int, C.c_int, FP]
         L.oracle_diffusion_init.argtypes = [FP, C.c_int, C.c_int, C.c_int,

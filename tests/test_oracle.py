@@ -346,3 +346,18 @@ def test_redblack_oracle_is_order_free():
             a[m] = v[m]
     assert np.array_equal(a.reshape(-1), o)
     assert not np.array_equal(o, g)
+
+
+def test_udt7_oracle_matches_reference_output():
+    """oracle_udt7_sweep with unit weights is the reference's
+    test_user-defined-type-7-pt (p = i, q = 0, one map over [1, N-1)^3;
+    integer sums below 2^24 are exact in any order): its "%f %f" dump of
+    (p, q) hashes to the reference oracle's output."""
+    n = 32
+    p = np.arange(n ** 3, dtype=np.float32)
+    q = np.zeros(n ** 3, np.float32)
+    lib().oracle_udt7_sweep(p, q, n, n, n, np.ones(7, np.float32))
+    rows = np.stack([p, q], 1).astype(np.float64)
+    dump = "".join("%f %f\n" % (a, b) for a, b in rows.tolist()).encode()
+    assert sha256(dump) == \
+        GOLD["system_tests"]["test_user-defined-type-7-pt"]["sha256"]
