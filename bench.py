@@ -527,7 +527,8 @@ def main() -> int:
     ps.PSGridFree(g2)
 
     # The same stencil over a grid of user-defined points (members p, q):
-    # q <- 7-point(p) over the interior, on the generic run kernel.
+    # q <- 7-point(p) over the interior; the translator splits the grid into
+    # one float grid per member and runs the 2.5D kernel over them.
     udt = Udt7()
     pq = np.zeros((N7 ** 3, 2), np.float32)
     pq[:, 0] = a
@@ -648,7 +649,8 @@ def main() -> int:
             "ms_per_sweep": round(udt_ms, 5),
             "hbm_frac": round(8 * N7 ** 3 / udt_ms / 1e6 / HBM_PEAK_GBS, 4),
             "note": "points (p, q) stored member by member; q <- 7-point(p) "
-                    "over the 510^3 interior of 512^3 (generic run kernel); "
+                    "over the 510^3 interior of 512^3 (2.5D run kernel over "
+                    "the member planes); "
                     "hbm_frac counts 8 B/pt (read p, write q)"},
         "himeno_19pt": {
             "grid": list(SHAPE19), "sweeps": 400,

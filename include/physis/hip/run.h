@@ -201,6 +201,44 @@ inline View DevView(const __PSGrid *g) {
   return v;
 }
 
+// The device view of one plane of a grid of user-defined points as a grid of
+// its own: component `comp` (row-major; 0 for a scalar member) of member
+// `member` (the runtime stores the points member by member, the components
+// of an array member one after another), with the grid's extents.
+template <typename View>
+inline View MemberView(__PSGrid *g, int member, int comp) {
+  View v;
+  constexpr int nd = (int)(sizeof(v.dim) / sizeof(v.dim[0]));
+  if (!g || !g->user || g->num_dims != nd) {
+    std::fprintf(stderr, "[physis-hip] stencil map on a %s grid where a "
+                 "%d-D grid of user-defined points is expected\n",
+                 g ? "mismatched" : "NULL", nd);
+    std::exit(EXIT_FAILURE);
+  }
+  for (int i = 0; i < nd; ++i) v.dim[i] = g->dim[i];
+  v.p = static_cast<decltype(v.p)>(__PSGridUserMember(g, member)) +
+        (long)comp * (long)g->num_elms;
+  return v;
+}
+
+// Checks a grid of user-defined points whose member planes a 2.5D run
+// kernel reads or writes as grids: 3-D, allocated, and within 32-bit
+// indexing.
+inline void CheckUserGrid3D(const __PSGrid *g) {
+  if (!g || !g->p || !g->user || g->num_dims != 3) {
+    std::fprintf(stderr, "[physis-hip] stencil map on a freed, NULL or "
+                 "mismatched grid where a 3-D grid of user-defined points is "
+                 "expected\n");
+    std::exit(EXIT_FAILURE);
+  }
+  if (g->num_elms >= (int64_t)PSINDEX_MAX) {
+    std::fprintf(stderr,
+                 "[physis-hip] grid of %lld elements exceeds 32-bit PSIndex\n",
+                 (long long)g->num_elms);
+    std::exit(EXIT_FAILURE);
+  }
+}
+
 inline void CheckEmitDomain(const __PSDomain &dom, const __PSGrid *g) {
   for (int d = 0; d < g->num_dims; ++d) {
     if (dom.local_max[d] <= dom.local_min[d]) return;  // empty domain

@@ -386,6 +386,316 @@ void Translator::AnalyzeGets(KernelInfo &k, const Stmt &s) {
     if (kid) AnalyzeGets(k, *kid);
 }
 
+namespace {
+
+// Splitting a grid of user-defined points into member grids.  The runtime
+// stores the points member by member (one dense plane per scalar member and
+// per component of an array member), so when a stencil touches a grid g of
+// points only through
+//   PSGridGet(g, i...).m, PSGridGet(g, i...).a[c0][c1]  (literal components)
+//   PSGridEmitUtype(g.m, v), PSGridEmitUtype(g.a[c0][c1], v)
+//   PSGridGet(g, i...), PSGridEmit(g, v)  (whole points of a struct of
+//                                          scalars, read and written member
+//                                          by member)
+//   PSGridDim(g, d)
+// each plane it touches is itself a 3-D grid of the member's scalar type, and
+// the stencil can run on the 2.5D kernel.  UserSplit checks a body for that
+// shape and rewrites it onto the plane grids g__m, g__a_c0_c1.
+struct UserSplit {
+  struct Part {
+    int member, comp;  // comp: row-major component of an array member, or 0
+    std::string name;
+  };
+  const std::string g, uname;
+  const std::vector<StructMember> &members;
+  bool ok = true;
+  std::vector<Part> used;  // first-use order
+
+  UserSplit(const std::string &name, const std::string &point_type,
+            const std::vector<StructMember> &ms)
+      : g(name), uname(point_type), members(ms) {}
+
+  static bool IsScalarType(const std::string &t) {
+    return t == "float" || t == "double" || t == "int" || t == "long";
+  }
+  static const Expr *Unp(const Expr *e) {
+    while (e && e->kind == EK::kParen) e = e->kids[0].get();
+    return e;
+  }
+  bool IsG(const Expr *e) const {
+    e = Unp(e);
+    return e && e->kind == EK::kIdent && e->text == g;
+  }
+  static const Expr *GetOf(const Expr *e) {
+    e = Unp(e);
+    const std::string *c = e ? CalleeName(*e) : nullptr;
+    return c && (*c == "PSGridGet" || *c == "PSGridGetPeriodic") &&
+                   e->kids.size() >= 2
+               ? e
+               : nullptr;
+  }
+  // e = B.m or B.a[c0][c1]... with literal components: the member access's
+  // base B (through *base) and the plane it names (*m, *comp); false if e is
+  // no such form or names no scalar plane of the point type.
+  bool Plane(const Expr *e, const Expr **base, int *m, int *comp) const {
+    std::vector<long> idx;
+    e = Unp(e);
+    while (e && e->kind == EK::kIndex) {
+      const Expr *i = Unp(e->kids[1].get());
+      if (!i || i->kind != EK::kLiteral) return false;
+      char *end = nullptr;
+      const long v = std::strtol(i->text.c_str(), &end, 10);
+      if (!end || *end || v < 0) return false;
+      idx.insert(idx.begin(), v);
+      e = Unp(e->kids[0].get());
+    }
+    if (!e || e->kind != EK::kMember || e->op != ".") return false;
+    for (size_t i = 0; i < members.size(); ++i) {
+      const StructMember &sm = members[i];
+      if (sm.name != e->text || !IsScalarType(sm.type) ||
+          sm.extents.size() != idx.size())
+        continue;
+      long c = 0;
+      for (size_t d = 0; d < idx.size(); ++d) {
+        if (idx[d] >= sm.extents[d]) return false;
+        c = c * sm.extents[d] + idx[d];
+      }
+      *base = e->kids[0].get();
+      *m = (int)i;
+      *comp = (int)c;
+      return true;
+    }
+    return false;
+  }
+  // the plane of a get of g: PSGridGet(g, i...).m / .a[c0]...
+  bool GetPlane(const Expr *e, const Expr **get, int *m, int *comp) const {
+    const Expr *base = nullptr;
+    if (!Plane(e, &base, m, comp)) return false;
+    *get = GetOf(base);
+    return *get && IsG((*get)->kids[1].get());
+  }
+  // the plane of g an emit names: g.m / g.a[c0]...
+  bool EmitPlane(const Expr *e, int *m, int *comp) const {
+    const Expr *base = nullptr;
+    return Plane(e, &base, m, comp) && IsG(base);
+  }
+  int Use(int m, int comp) {
+    for (size_t i = 0; i < used.size(); ++i)
+      if (used[i].member == m && used[i].comp == comp) return (int)i;
+    std::string n = g + "__" + members[m].name;
+    if (!members[m].extents.empty()) {
+      int c = comp;
+      std::vector<int> idx(members[m].extents.size());
+      for (size_t d = idx.size(); d-- > 0;) {
+        idx[d] = c % members[m].extents[d];
+        c /= members[m].extents[d];
+      }
+      for (int i : idx) n += "_" + std::to_string(i);
+    }
+    used.push_back({m, comp, n});
+    return (int)used.size() - 1;
+  }
+  // A whole point is read or written member by member: every member must
+  // be a scalar, and the expressions that are repeated per member (the
+  // indices of a get, the value of an emit) must be free of side effects.
+  bool Whole() {
+    for (auto &m : members)
+      if (!m.extents.empty() || !IsScalarType(m.type)) return false;
+    for (size_t i = 0; i < members.size(); ++i) Use((int)i, 0);
+    return true;
+  }
+  static bool Pure(const Expr &e) {
+    switch (e.kind) {
+      case EK::kCall: {
+        const std::string *c = CalleeName(e);
+        if (!c || *c != "PSGridDim") return false;
+        break;
+      }
+      case EK::kUnary:
+      case EK::kPostfix:
+        if (e.op == "++" || e.op == "--") return false;
+        break;
+      case EK::kBinary:
+        if (!e.op.empty() && e.op.back() == '=' && e.op != "==" &&
+            e.op != "!=" && e.op != "<=" && e.op != ">=")
+          return false;
+        break;
+      case EK::kRaw:
+        return false;
+      default:
+        break;
+    }
+    for (const auto &kid : e.kids)
+      if (kid && !Pure(*kid)) return false;
+    return true;
+  }
+  static ExprP Clone(const Expr &e) {
+    auto c = std::make_unique<Expr>(e.kind);
+    c->op = e.op;
+    c->text = e.text;
+    c->type = e.type;
+    c->line = e.line;
+    for (const auto &kid : e.kids) c->kids.push_back(kid ? Clone(*kid) : nullptr);
+    return c;
+  }
+  static ExprP Ident(const std::string &n) {
+    auto e = std::make_unique<Expr>(EK::kIdent);
+    e->text = n;
+    return e;
+  }
+  static ExprP Call(const std::string &callee) {
+    auto e = std::make_unique<Expr>(EK::kCall);
+    e->kids.push_back(Ident(callee));
+    return e;
+  }
+
+  void Check(const Expr &e) {
+    if (!ok) return;
+    const Expr *get = nullptr;
+    int m, comp;
+    if ((e.kind == EK::kMember || e.kind == EK::kIndex) &&
+        GetPlane(&e, &get, &m, &comp)) {
+      // PSGridGet(g, i...).m / .a[c]...: a get of one plane
+      Use(m, comp);
+      for (size_t i = 2; i < get->kids.size(); ++i) Check(*get->kids[i]);
+      return;
+    }
+    if (const std::string *c = CalleeName(e)) {
+      if (*c == "PSGridEmitUtype" && e.kids.size() == 3 &&
+          EmitPlane(e.kids[1].get(), &m, &comp)) {
+        Use(m, comp);
+        Check(*e.kids[2]);
+        return;
+      }
+      if (*c == "PSGridDim" && e.kids.size() == 3 && IsG(e.kids[1].get())) {
+        Check(*e.kids[2]);
+        return;
+      }
+      if (*c == "PSGridEmit" && e.kids.size() == 3 && IsG(e.kids[1].get())) {
+        if (!Pure(*e.kids[2]) || !Whole()) {
+          ok = false;
+          return;
+        }
+        Check(*e.kids[2]);
+        return;
+      }
+    }
+    if (const Expr *call = GetOf(&e)) {
+      if (IsG(call->kids[1].get())) {  // a whole point read
+        for (size_t i = 2; i < call->kids.size(); ++i)
+          if (!Pure(*call->kids[i])) ok = false;
+        if (!ok || !Whole()) {
+          ok = false;
+          return;
+        }
+        for (size_t i = 2; i < call->kids.size(); ++i) Check(*call->kids[i]);
+        return;
+      }
+    }
+    if (e.kind == EK::kIdent && e.text == g) {
+      ok = false;  // g in any other use: a helper call, an alias, a member
+      return;      // access that names no scalar plane
+    }
+    for (const auto &kid : e.kids)
+      if (kid) Check(*kid);
+  }
+  void Check(const Stmt &st) {
+    for (const auto &d : st.decls) {
+      if (d.name == g) ok = false;  // shadowed
+      if (d.init) Check(*d.init);
+    }
+    for (const Expr *e : {st.expr.get(), st.cond.get(), st.step.get()})
+      if (e) Check(*e);
+    if (st.init) Check(*st.init);
+    for (const auto &kid : st.kids)
+      if (kid) Check(*kid);
+  }
+
+  void Rewrite(ExprP &e) {
+    Expr &x = *e;
+    const Expr *get = nullptr;
+    int m, comp;
+    if ((x.kind == EK::kMember || x.kind == EK::kIndex) &&
+        GetPlane(&x, &get, &m, &comp)) {
+      ExprP call = Clone(*get);
+      call->kids[1] = Ident(used[Use(m, comp)].name);
+      for (size_t i = 2; i < call->kids.size(); ++i) Rewrite(call->kids[i]);
+      e = std::move(call);
+      return;
+    }
+    if (const std::string *c = CalleeName(x)) {
+      if (*c == "PSGridEmitUtype" && x.kids.size() == 3 &&
+          EmitPlane(x.kids[1].get(), &m, &comp)) {
+        x.kids[0]->text = "PSGridEmit";
+        x.kids[1] = Ident(used[Use(m, comp)].name);
+        Rewrite(x.kids[2]);
+        return;
+      }
+      if (*c == "PSGridDim" && x.kids.size() == 3 && IsG(x.kids[1].get())) {
+        x.kids[1] = Ident(used[0].name);
+        Rewrite(x.kids[2]);
+        return;
+      }
+      if (*c == "PSGridEmit" && x.kids.size() == 3 && IsG(x.kids[1].get())) {
+        // PSGridEmit(g, v) -> (PSGridEmit(g__m0, (v).m0), ...)
+        Rewrite(x.kids[2]);
+        ExprP chain;
+        for (size_t m = 0; m < members.size(); ++m) {
+          ExprP call = Call("PSGridEmit");
+          call->kids.push_back(Ident(used[Use((int)m, 0)].name));
+          auto val = std::make_unique<Expr>(EK::kParen);
+          val->kids.push_back(Clone(*x.kids[2]));
+          auto mem = std::make_unique<Expr>(EK::kMember);
+          mem->op = ".";
+          mem->text = members[m].name;
+          mem->kids.push_back(std::move(val));
+          call->kids.push_back(std::move(mem));
+          if (!chain) {
+            chain = std::move(call);
+          } else {
+            auto comma = std::make_unique<Expr>(EK::kBinary);
+            comma->op = ",";
+            comma->kids.push_back(std::move(chain));
+            comma->kids.push_back(std::move(call));
+            chain = std::move(comma);
+          }
+        }
+        auto paren = std::make_unique<Expr>(EK::kParen);
+        paren->kids.push_back(std::move(chain));
+        e = std::move(paren);
+        return;
+      }
+    }
+    if (const Expr *get = GetOf(&x)) {
+      if (IsG(get->kids[1].get())) {
+        // a whole point read: __PSUdtMake_<type>(PSGridGet(g__m0, i...), ...)
+        ExprP make = Call("__PSUdtMake_" + uname);
+        for (size_t m = 0; m < members.size(); ++m) {
+          ExprP one = Clone(*get);
+          one->kids[1] = Ident(used[Use((int)m, 0)].name);
+          for (size_t i = 2; i < one->kids.size(); ++i) Rewrite(one->kids[i]);
+          make->kids.push_back(std::move(one));
+        }
+        e = std::move(make);
+        return;
+      }
+    }
+    for (auto &kid : x.kids)
+      if (kid) Rewrite(kid);
+  }
+  void Rewrite(Stmt &st) {
+    for (auto &d : st.decls)
+      if (d.init) Rewrite(d.init);
+    for (ExprP *e : {&st.expr, &st.cond, &st.step})
+      if (*e) Rewrite(*e);
+    if (st.init) Rewrite(*st.init);
+    for (auto &kid : st.kids)
+      if (kid) Rewrite(*kid);
+  }
+};
+
+}  // namespace
+
 void Translator::AnalyzeKernel(KernelInfo &k) {
   const Function &f = *k.fn;
   k.ret_type = Strip(f.ret_type);
@@ -423,6 +733,32 @@ void Translator::AnalyzeKernel(KernelInfo &k) {
     } else {
       k.order.push_back({'s', (int)k.scalars.size()});
       k.scalars.push_back({f.params[i].type, f.params[i].name});
+    }
+  }
+  // 3-D grids of user-defined points used only member-wise become member
+  // grids (the stencil body is rewritten onto them)
+  if (k.ndims == 3) {
+    const size_t nreal = k.grids.size();
+    for (size_t gi = 0; gi < nreal; ++gi) {
+      if (!k.grids[gi].user || k.grids[gi].dims != 3) continue;
+      UserSplit us(k.grids[gi].name, k.grids[gi].uname, *k.grids[gi].members);
+      us.Check(*f.body);
+      if (!us.ok || us.used.empty()) continue;
+      us.Rewrite(*const_cast<Function &>(f).body);
+      k.grids[gi].split = true;
+      for (const auto &part : us.used) {
+        const StructMember &sm = (*k.grids[gi].members)[part.member];
+        GridParam v;
+        v.name = part.name;
+        v.dims = 3;
+        v.elem = sm.type;
+        v.ps_type = PsTypeOf(sm.type);
+        v.dev_view = "__PSGrid3D" + Cap(sm.type) + "_dev";
+        v.member_of = (int)gi;
+        v.member_index = part.member;
+        v.member_comp = part.comp;
+        k.grids.push_back(v);
+      }
     }
   }
   k.index_rebound = RebindsIndex(k, *f.body);
@@ -738,7 +1074,7 @@ void Translator::Decide(KernelInfo &k) {
   k.fast = k.ndims == 3;
   if (!k.fast) k.slow_reason = "not a 3-D stencil";
   for (auto &g : k.grids)
-    if (g.user && k.fast) {
+    if (g.user && !g.split && k.fast) {
       k.fast = false;
       k.slow_reason = "grid " + g.name + " holds user-defined points";
     }
@@ -1063,6 +1399,19 @@ std::string Translator::EmitUserType(const std::string &uname) {
         << "[c * v.n + off] = " << access(m, "r") << "[c];\n";
   }
   o << "}\n\n";
+  // a point from its members (the member-split run kernels read whole points
+  // of a struct of scalars this way)
+  bool scalars = true;
+  for (auto &m : *ms) scalars = scalars && m.extents.empty();
+  if (scalars) {
+    o << "__device__ __forceinline__ " << elem << " __PSUdtMake_" << uname << "(";
+    for (size_t i = 0; i < ms->size(); ++i)
+      o << (i ? ", " : "") << (*ms)[i].type << " m" << i;
+    o << ") {\n  " << elem << " r;\n";
+    for (size_t i = 0; i < ms->size(); ++i)
+      o << "  r." << (*ms)[i].name << " = m" << i << ";\n";
+    o << "  return r;\n}\n\n";
+  }
   return o.str();
 }
 
@@ -1078,6 +1427,26 @@ void Translator::RewriteKernelStmt(const KernelInfo &k, Stmt &s) {
 }
 
 // ============================================================ code emission
+namespace {
+
+// The host handle (a field of the stencil struct s) behind a kernel grid:
+// the grid itself, or for a member grid its grid of user-defined points.
+std::string HostGrid(const KernelInfo &k, const GridParam &g) {
+  return "s." + (g.member_of >= 0 ? k.grids[g.member_of].name : g.name);
+}
+
+// The launch argument (device view) of a kernel grid.
+std::string ViewArg(const KernelInfo &k, const GridParam &g) {
+  if (g.member_of >= 0)
+    return "physis_hip::MemberView<" + g.dev_view + ">(" + HostGrid(k, g) +
+           ", " + std::to_string(g.member_index) + ", " +
+           std::to_string(g.member_comp) + ")";
+  if (g.user) return "__PSUdtViewOf_" + g.uname + "(s." + g.name + ")";
+  return "physis_hip::DevView<" + g.dev_view + ">(s." + g.name + ")";
+}
+
+}  // namespace
+
 std::string Translator::EmitStencilStruct(const KernelInfo &k) {
   std::ostringstream o;
   const std::string &n = k.name;
@@ -1146,6 +1515,10 @@ std::string Translator::EmitContext(const KernelInfo &k) {
     << "):\n";
   for (auto &g : k.grids) {
     o << "//   " << g.name << "  ";
+    if (g.split) {
+      o << "split into one grid per member it uses\n";
+      continue;
+    }
     if (g.mode == GridParam::kStaged)
       o << "staged x[-" << g.hx << "," << g.hx << "] y[-" << g.hy << ","
         << g.hy << "] z[" << g.zl << "," << g.zr << "]"
@@ -1177,6 +1550,7 @@ std::string Translator::EmitContext(const KernelInfo &k) {
                     : "no (" + k.fuse_why + ")") << "\n";
   o << "struct __PSCtx_" << k.name << " {\n";
   for (auto &g : k.grids) {
+    if (g.split) continue;  // its member grids stand in for it
     if (g.user) {
       o << "  " << g.dev_view << " " << g.name << "_u;\n";
       continue;
@@ -1239,7 +1613,8 @@ std::string Translator::EmitFastRunKernel(const KernelInfo &k) {
   const bool red = k.ret_type != "void";
   const std::string RT = red ? k.ret_type : "float";
   std::vector<std::string> params = {"__PSDomain dom"};
-  for (auto &g : k.grids) params.push_back(g.dev_view + " " + g.name);
+  for (auto &g : k.grids)
+    if (!g.split) params.push_back(g.dev_view + " " + g.name);
   for (auto &s : k.scalars) params.push_back(s.type + " " + s.name);
   params.push_back("physis_hip::RunShape sh");
   params.push_back(RT + " *__restrict__ __red");
@@ -1328,7 +1703,9 @@ std::string Translator::EmitFastRunKernel(const KernelInfo &k) {
   o << "    __syncthreads();\n";
   o << "    __PSCtx_" << n << " ctx;\n";
   for (auto &g : k.grids)
-    if (g.user)
+    if (g.split)
+      continue;
+    else if (g.user)
       o << "    ctx." << g.name << "_u = " << g.name << ";\n";
     else
     o << "    ctx." << g.name << "_p = " << g.name << ".p;\n"
@@ -1480,7 +1857,8 @@ std::string Translator::EmitGenericRunKernel(const KernelInfo &k) {
   const bool red = k.ret_type != "void";
   const std::string RT = red ? k.ret_type : "float";
   std::vector<std::string> params = {"__PSDomain dom"};
-  for (auto &g : k.grids) params.push_back(g.dev_view + " " + g.name);
+  for (auto &g : k.grids)
+    if (!g.split) params.push_back(g.dev_view + " " + g.name);
   for (auto &s : k.scalars) params.push_back(s.type + " " + s.name);
   params.push_back(RT + " *__restrict__ __red");
   if (k.used_rb) params.push_back("int __rb");
@@ -1501,7 +1879,9 @@ std::string Translator::EmitGenericRunKernel(const KernelInfo &k) {
   o << "  if (x < dom.local_max[0]" << (k.ndims >= 2 ? " && y < dom.local_max[1]" : "")
     << ") {\n    __PSCtx_" << n << " ctx;\n";
   for (auto &g : k.grids)
-    if (g.user)
+    if (g.split)
+      continue;
+    else if (g.user)
       o << "    ctx." << g.name << "_u = " << g.name << ";\n";
     else
     o << "    ctx." << g.name << "_p = " << g.name << ".p;\n"
@@ -1852,9 +2232,7 @@ std::string Translator::EmitLauncher(const KernelInfo &k) {
   const std::string RT = red ? k.ret_type : "float";
   std::vector<std::string> args = {"s.dom"};
   for (auto &g : k.grids)
-    args.push_back(g.user ? "__PSUdtViewOf_" + g.uname + "(s." + g.name + ")"
-                          : "physis_hip::DevView<" + g.dev_view + ">(s." +
-                                g.name + ")");
+    if (!g.split) args.push_back(ViewArg(k, g));
   for (auto &s : k.scalars) args.push_back("s." + s.name);
   std::string A = Join(args, ", ");
   if (k.fast) {
@@ -1886,7 +2264,8 @@ std::string Translator::EmitLauncher(const KernelInfo &k) {
     std::vector<std::string> wgrids;
     for (auto &g : k.grids) {
       if (!(g.read || g.emitted) || g.user) continue;
-      wgrids.push_back("s." + g.name);
+      if (std::find(wgrids.begin(), wgrids.end(), HostGrid(k, g)) == wgrids.end())
+        wgrids.push_back(HostGrid(k, g));
       if (g.mode == GridParam::kStaged)
         margin = std::max(margin, std::max(1 - g.zl, g.zr + 1));
     }
@@ -1924,21 +2303,31 @@ std::string Translator::EmitLauncher(const KernelInfo &k) {
       << ", -1>), grid, block, 0, stream, " << A << ", sh, (" << RT
       << " *)NULL" << RBA << ");\n  }\n  }\n  return sh.blocks;\n}\n\n";
     // vector width: every 3-D grid's x extent a multiple of 4
-    std::vector<std::string> conds;
-    for (auto &g : k.grids)
-      if (g.read || g.emitted) conds.push_back("s." + g.name + "->dim[0] % 4 == 0");
+    // (the member grids of one grid of user-defined points share its checks)
+    std::vector<std::string> conds, checks, emit_checks;
+    auto add = [](std::vector<std::string> &v, const std::string &x) {
+      if (std::find(v.begin(), v.end(), x) == v.end()) v.push_back(x);
+    };
+    for (auto &g : k.grids) {
+      if (g.read || g.emitted) {
+        add(conds, HostGrid(k, g) + "->dim[0] % 4 == 0");
+        add(checks, g.member_of >= 0
+                        ? "physis_hip::CheckUserGrid3D(" + HostGrid(k, g) + ")"
+                        : "physis_hip::CheckGrid3D(s." + g.name + ", " +
+                              g.ps_type + ")");
+      }
+      if (g.emitted)
+        add(emit_checks,
+            "physis_hip::CheckEmitDomain(s.dom, " + HostGrid(k, g) + ")");
+    }
     std::string vec = conds.empty() ? "true" : Join(conds, " && ");
     o << "static int __PSBlocks_" << n << "(const struct __PSStencil_" << n
       << " &s) {\n  return (" << vec << ") ? __PSShape_" << n
       << "<4>(s).blocks : __PSShape_" << n << "<1>(s).blocks;\n}\n\n";
     o << "static int __PSRun_" << n << "(const struct __PSStencil_" << n
       << " &s, hipStream_t stream, void *red) {\n";
-    for (auto &g : k.grids)
-      if (g.read || g.emitted)
-        o << "  physis_hip::CheckGrid3D(s." << g.name << ", " << g.ps_type << ");\n";
-    for (auto &g : k.grids)
-      if (g.emitted)
-        o << "  physis_hip::CheckEmitDomain(s.dom, s." << g.name << ");\n";
+    for (auto &c : checks) o << "  " << c << ";\n";
+    for (auto &c : emit_checks) o << "  " << c << ";\n";
     o << "  return (" << vec << ") ? __PSLaunch_" << n
       << "<4>(s, stream, red) : __PSLaunch_" << n << "<1>(s, stream, red);\n}\n\n";
   } else {
@@ -1956,7 +2345,8 @@ std::string Translator::EmitLauncher(const KernelInfo &k) {
       << " &s, hipStream_t stream, void *red) {\n";
     for (auto &g : k.grids)
       if (g.emitted)
-        o << "  physis_hip::CheckEmitDomain(s.dom, s." << g.name << ");\n";
+        o << "  physis_hip::CheckEmitDomain(s.dom, " << HostGrid(k, g)
+          << ");\n";
     if (k.ndims == 3)
       o << "  if (s.dom.local_max[2] <= s.dom.local_min[2]) return 0;\n";
     o << "  const dim3 grid = __PSGridDim_" << n << "(s);\n"

@@ -55,6 +55,14 @@ struct GridParam {
   bool user = false;
   std::string uname;
   const std::vector<StructMember> *members = nullptr;
+  // A user grid whose stencil reads and emits it only plane by plane (a
+  // scalar member, a literal component of an array member, or whole points
+  // of a struct of scalars member by member) is split: split = true on it,
+  // and each plane it uses becomes a grid of its own (member_of = the user
+  // grid's index, member_index = the member's, member_comp = the component
+  // of an array member, row-major).
+  bool split = false;
+  int member_of = -1, member_index = -1, member_comp = 0;
 };
 
 struct ScalarParam {

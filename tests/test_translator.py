@@ -9,9 +9,9 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PHYSISC = os.path.join(ROOT, "physis_amd", "bin", "physisc-hip")
-SOURCES = [os.path.join(ROOT, "physis_amd", "programs", "diffusion7.c"),
-           os.path.join(ROOT, "physis_amd", "programs", "himeno.c"),
-           os.path.join(ROOT, "tests", "programs", "systests.c")]
+SOURCES = [os.path.join(ROOT, "physis_amd", "programs", f)
+           for f in ("diffusion7.c", "himeno.c", "redblack7.c", "udt7.c")] + \
+          [os.path.join(ROOT, "tests", "programs", "systests.c")]
 
 
 def translate(src, text=None):
@@ -180,6 +180,81 @@ def test_user_point_type_lowering():
     # grids of the type are created from the type info
     assert "physis_hip::NewUserGrid(&__PSTypeInfo_Cell, 3, 8, 8, 8)" in code
     assert "generic run kernel: grid g holds user-defined points" in code
+
+
+SPLIT_PROG = """#include "physis/physis.h"
+struct Cplx {
+  float r;
+  float i;
+};
+DeclareGrid3D(Cplx, struct Cplx);
+struct Arr {
+  float p[3][2];
+  int tag;
+};
+DeclareGrid3D(Arr, struct Arr);
+static struct Cplx twice(struct Cplx a) {
+  struct Cplx b = {2 * a.r, 2 * a.i};
+  return b;
+}
+static void whole(const int x, const int y, const int z, PSGrid3DCplx g1,
+                  PSGrid3DCplx g2) {
+  struct Cplx a = PSGridGet(g1, x + 1, y, z);
+  struct Cplx b = PSGridGet(g1, x, y, z - 1);
+  struct Cplx v = {a.r + b.i, a.i - PSGridGet(g1, x, y + 1, z).r};
+  PSGridEmit(g2, v);
+}
+static void parts(const int x, const int y, const int z, PSGrid3DArr g) {
+  float s = PSGridGet(g, x - 1, y, z).p[2][1] + PSGridGet(g, x, y, z).p[0][0];
+  PSGridEmitUtype(g.p[1][0], s + PSGridGet(g, x, y, z + 1).tag);
+}
+static void impure(const int x, const int y, const int z, PSGrid3DCplx g1,
+                   PSGrid3DCplx g2) {
+  PSGridEmit(g2, twice(PSGridGet(g1, x, y, z)));
+}
+static void looped(const int x, const int y, const int z, PSGrid3DArr g) {
+  int i;
+  float s = 0;
+  for (i = 0; i < 3; ++i) s += PSGridGet(g, x, y, z).p[i][0];
+  PSGridEmitUtype(g.p[0][1], s);
+}
+void run(PSGrid3DCplx g1, PSGrid3DCplx g2, PSGrid3DArr g) {
+  PSDomain3D d = PSDomain3DNew(1, 7, 1, 7, 1, 7);
+  PSStencilRun(PSStencilMap(whole, d, g1, g2), PSStencilMap(parts, d, g),
+               PSStencilMap(impure, d, g1, g2), PSStencilMap(looped, d, g));
+}
+"""
+
+
+def test_user_point_grids_split_into_member_grids():
+    """A grid of user-defined points touched plane by plane (scalar members,
+    literal components of array members, whole points of a struct of
+    scalars) runs on the 2.5D kernel over one grid per plane."""
+    rc, code, err = translate(None, SPLIT_PROG)
+    assert rc == 0, err
+    assert "Stencil analysis of whole (2.5D run kernel)" in code
+    assert "g1__r  staged x[-1,1] y[-1,1] z[-1,0] non-periodic" in code
+    assert "g1__i  staged x[-1,1] y[-0,0] z[-1,0] non-periodic" in code
+    # a whole point read member by member, a whole point emitted likewise
+    assert "__PSUdtMake_Cplx(__ctx.g1__r[" in code
+    assert "(__ctx.g2__r_v = ((v).r), __ctx.g2__r_e = true)" in code
+    assert "(__ctx.g2__i_v = ((v).i), __ctx.g2__i_e = true)" in code
+    # component planes: p[2][1] is component 5 of member 0, p[1][0] is 2
+    assert "Stencil analysis of parts (2.5D run kernel)" in code
+    assert "physis_hip::MemberView<__PSGrid3DFloat_dev>(s.g, 0, 5)" in code
+    assert "physis_hip::MemberView<__PSGrid3DFloat_dev>(s.g, 0, 2)" in code
+    assert "physis_hip::MemberView<__PSGrid3DInt_dev>(s.g, 1, 0)" in code
+    assert "physis_hip::CheckUserGrid3D(s.g);" in code
+    # a point passed to a function, a component under a loop index: whole
+    # points through the generic kernel (g1 of impure is still read member
+    # by member; the emitted value, a call, is not repeated per member)
+    assert ("Stencil analysis of impure (generic run kernel: grid g2 holds "
+            "user-defined points)") in code
+    assert "__PSStencilRunGeneric_impure<-1>), grid, dim3(256), 0, stream, " \
+        "s.dom, __PSUdtViewOf_Cplx(s.g2), physis_hip::MemberView<" \
+        "__PSGrid3DFloat_dev>(s.g1, 0, 0)" in code
+    assert ("Stencil analysis of looped (generic run kernel: grid g holds "
+            "user-defined points)") in code
 
 
 def test_user_point_member_must_be_indexed_to_a_scalar():
