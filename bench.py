@@ -32,6 +32,7 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E peak
 BYTES_PER_PT_7 = 8.0           # read u, write u' (diffusion3d.h:97-100)
 BYTES_PER_PT_19 = 56.0         # 12 coefficient streams + p + p'
 N7 = 512
+N2D = 16384
 SHAPE19 = (512, 256, 256)
 
 
@@ -430,7 +431,8 @@ def main() -> int:
     from physis_amd import runtime as ps
     from physis_amd.configs import (DIFFUSION_COEF, diffusion_input,
                                     himeno_inputs)
-    from physis_amd.programs import Diffusion7, Himeno, RedBlack7, Udt7
+    from physis_amd.programs import (Diffusion7, Himeno, RedBlack7, Stencil2D,
+                                     Udt7)
 
     ps.PSInit()
     hip = Hip(ps.hip_stream())
@@ -548,6 +550,24 @@ def main() -> int:
     udt_ms = hip.ms(e0, e1) / udt_sweeps
     udt.free(ug)
 
+    # 2-D 5- and 9-point maps over 16384^2 (the 2.5D kernel over the grids'
+    # rows view), 20 pairs of sweeps each.
+    s2 = Stencil2D()
+    g1 = ps.PSGridNew((N2D, N2D))
+    g2 = ps.PSGridNew((N2D, N2D))
+    ps.PSGridCopyin(g1, np.random.default_rng(2).standard_normal(
+        (N2D, N2D)).astype(np.float32))
+    s2d_ms = {}
+    for pts in (5, 9):
+        s2.run(g1, g2, pts, 1, 1.0 / pts)
+        e0, e1 = hip.event(), hip.event()
+        hip.record(e0)
+        s2.run(g1, g2, pts, 20, 1.0 / pts)
+        hip.record(e1)
+        s2d_ms[pts] = hip.ms(e0, e1) / 40
+    ps.PSGridFree(g1)
+    ps.PSGridFree(g2)
+
     # 19-point Himeno, 512x256x256, 400 sweeps with per-sweep gosa.
     log("generating Himeno 512x256x256 inputs")
     coefs, p = himeno_inputs(*SHAPE19, seed=0)
@@ -657,6 +677,16 @@ def main() -> int:
                     "over the 510^3 interior of 512^3 (2.5D run kernel over "
                     "the member planes); "
                     "hbm_frac counts 8 B/pt (read p, write q)"},
+        "stencil2d": {
+            "grid": [N2D, N2D],
+            **{f"{pts}pt": {
+                "value": round((N2D - 2) ** 2 / (ms * 1e-3) / 1e9, 2),
+                "unit": "Gpts/s", "ms_per_sweep": round(ms, 5),
+                "hbm_frac": round(8 * N2D ** 2 / ms / 1e6 / HBM_PEAK_GBS, 4)}
+               for pts, ms in s2d_ms.items()},
+            "note": "2-D maps a -> b -> a over the interior, on the 2.5D "
+                    "kernel over the rows view (nx x 1 x ny); hbm_frac "
+                    "counts 8 B/pt per sweep (read a, write b)"},
         "himeno_19pt": {
             "grid": list(SHAPE19), "sweeps": 400,
             "value": round(n19 * 400 / t19 / 1e9, 2), "unit": "Gpts/s",

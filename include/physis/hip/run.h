@@ -92,6 +92,21 @@ inline int WindowChunkLimit(std::initializer_list<const __PSGrid *> grids,
   return m;
 }
 
+// The same reach for a 2-D grid in the rows view (nx x 1 x ny): z chunks
+// are chunks of rows.
+inline int WindowChunkLimitRows(std::initializer_list<const __PSGrid *> grids,
+                                int margin) {
+  int m = INT32_MAX;
+  for (const __PSGrid *g : grids) {
+    if (!g) continue;
+    __PSGrid row = *g;
+    row.num_dims = 1;  // one row of dim[0] elements per "plane"
+    const int c = WindowChunkLimit(&row, margin);
+    m = c < m ? c : m;
+  }
+  return m;
+}
+
 // Tiles a domain into TX x TY column tiles and picks the number of z chunks.
 // Each block costs (chunk + 2) plane steps (two prologue planes), blocks run
 // in waves of `resident` blocks; the chunk count minimising
@@ -199,6 +214,36 @@ inline View DevView(const __PSGrid *g) {
   for (int i = 0; i < nd; ++i) v.dim[i] = g->dev->dim[i];
   v.p = static_cast<decltype(v.p)>(g->dev->p);
   return v;
+}
+
+// The rows view of a 2-D grid: the 3-D view nx x 1 x ny of the same
+// memory (equal linear offsets), in which a 2-D stencil runs on the 2.5D
+// path, marching y as z.
+template <typename View>
+inline View RowView(const __PSGrid *g) {
+  View v;
+  static_assert(sizeof(v.dim) / sizeof(v.dim[0]) == 3, "a 3-D view");
+  if (!g || !g->dev || g->num_dims != 2) {
+    std::fprintf(stderr, "[physis-hip] stencil map on a %s grid where a "
+                 "2-D grid is expected\n", g ? "mismatched" : "NULL");
+    std::exit(EXIT_FAILURE);
+  }
+  v.dim[0] = g->dev->dim[0];
+  v.dim[1] = 1;
+  v.dim[2] = g->dev->dim[1];
+  v.p = static_cast<decltype(v.p)>(g->dev->p);
+  return v;
+}
+// The domain of a 2-D map in the rows view: [x0, x1) x [0, 1) x [y0, y1).
+inline __PSDomain RowDomain(const __PSDomain &d) {
+  __PSDomain r = d;
+  r.min[2] = d.min[1];
+  r.max[2] = d.max[1];
+  r.local_min[2] = d.local_min[1];
+  r.local_max[2] = d.local_max[1];
+  r.min[1] = r.local_min[1] = 0;
+  r.max[1] = r.local_max[1] = 1;
+  return r;
 }
 
 // The device view of one plane of a grid of user-defined points as a grid of

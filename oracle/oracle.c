@@ -263,6 +263,31 @@ void oracle_redblack7_run(float *g, int nx, int ny, int nz, int count,
 #undef G
 }
 
+/* ------------------------------------------------- 2-D 5- and 9-point maps */
+/* sweeps of physis_amd/programs/stencil2d.c, alternating a -> b and b -> a
+ * (the first from a), each over [1, nx-1) x [1, ny-1) in the REFERENCE
+ * target's loop order (y, then x): v = the points summed in the order of
+ * test_5-pt-2d.c:13-18 / test_9-pt-2d.c:13-22, out = v * w.  Points outside
+ * the domain keep their value. */
+void oracle_stencil2d_run(float *a, float *b, int nx, int ny, int points,
+                          float w, int sweeps) {
+  for (int s = 0; s < sweeps; ++s) {
+    const float *g = (s & 1) ? b : a;
+    float *o = (s & 1) ? a : b;
+#define G(i, j) g[(long)(j) * nx + (i)]
+    for (int y = 1; y < ny - 1; ++y)
+      for (int x = 1; x < nx - 1; ++x) {
+        float v = G(x, y) + G(x - 1, y) + G(x + 1, y) + G(x, y - 1) +
+                  G(x, y + 1);
+        if (points == 9)
+          v = v + G(x - 1, y - 1) + G(x + 1, y - 1) + G(x - 1, y + 1) +
+              G(x + 1, y + 1);
+        o[(long)y * nx + x] = v * w;
+      }
+#undef G
+  }
+}
+
 /* ------------------------------------ 7-point stencil on user-defined points */
 /* One sweep of physis_amd/programs/udt7.c over [1, n-1) of every axis:
  * q[c] = cc p[c] + cw p[x-1] + ce p[x+1] + cs p[y+1] + cn p[y-1] + cb p[z-1]

@@ -103,6 +103,12 @@ int oracle_himeno_run_omp(float *p0, float *p1, const float *const *coef,
 void oracle_redblack7_run(float *g, int nx, int ny, int nz, int count,
                           const float *coef);
 
+/* ---- 2-D 5- and 9-point maps (physis_amd/programs/stencil2d.c): `sweeps`
+ * sweeps alternating a -> b, b -> a over [1, nx-1) x [1, ny-1); out = (sum of
+ * the points in the reference tests' order) * w. */
+void oracle_stencil2d_run(float *a, float *b, int nx, int ny, int points,
+                          float w, int sweeps);
+
 /* ---- 7-point stencil on user-defined points (physis_amd/programs/udt7.c):
  * one sweep q <- weighted 7-point sum of p over [1, n-1)^3; coef = ce cw cn
  * cs ct cb cc. */

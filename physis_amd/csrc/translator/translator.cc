@@ -1008,7 +1008,8 @@ void Translator::AnalyzeFusion(KernelInfo &k) {
   k.fuse_domain = false;
   auto no = [&](const std::string &w) { k.fuse_why = w; };
   if (!cfg_.two_sweep) return no("HIP_TWO_SWEEP = false");
-  if (!k.fast || k.ndims != 3) return no("not a 2.5D run kernel");
+  if (!k.fast) return no("not a 2.5D run kernel");
+  if (k.rows2d) return no("a 2-D map");
   if (k.index_rebound) return no("an index parameter is rebound");
   if (k.used_rb) return no("used in red-black maps");
   int in = -1, out = -1, direct = 0;
@@ -1072,7 +1073,18 @@ void Translator::Decide(KernelInfo &k) {
   for (int a = 1; a < k.ndims; ++a)
     k.compares_axis[a] = ComparesIdent(*k.fn->body, k.index[a]);
   k.fast = k.ndims == 3;
-  if (!k.fast) k.slow_reason = "not a 3-D stencil";
+  if (k.ndims == 2) {
+    // every grid the kernel touches 2-D and of a primitive type: the rows
+    // view (grids as nx x 1 x ny, y marched as z) runs it on the 2.5D path
+    bool rows = true;
+    for (auto &g : k.grids)
+      if ((g.read || g.emitted) && (g.dims != 2 || g.user)) rows = false;
+    k.fast = k.rows2d = rows;
+  }
+  if (!k.fast)
+    k.slow_reason = k.ndims == 2 ? "a 2-D stencil over grids that are not all "
+                                   "2-D grids of a primitive type"
+                                 : "not a 2-D or 3-D stencil";
   for (auto &g : k.grids)
     if (g.user && !g.split && k.fast) {
       k.fast = false;
@@ -1110,11 +1122,16 @@ void Translator::Decide(KernelInfo &k) {
     // neighbouring tiles and neighbouring z chunks are read race-free.
   }
   for (auto &g : k.grids)
-    if (g.dims != 3 && (g.read || g.emitted)) {
+    if (k.fast && g.dims != (k.rows2d ? 2 : 3) && (g.read || g.emitted)) {
       k.fast = false;
       k.slow_reason = "grid " + g.name + " is not 3-D";
     }
   if (!k.fast) return;
+  if (k.rows2d)
+    for (auto &g : k.grids) {
+      for (auto &o : g.offsets) o = {o[0], 0, o[1]};
+      if (g.dims == 2) g.dev_view = "__PSGrid3D" + Cap(g.elem) + "_dev";
+    }
   for (auto &g : k.grids) {
     bool nonzero = false, any_periodic = false;
     for (size_t i = 0; i < g.offsets.size(); ++i) {
@@ -1203,9 +1220,10 @@ void Translator::RewriteKernelExpr(const KernelInfo &k, ExprP &e) {
   if (get || getp) {
     const GridParam &g = *k.Grid(e->kids[1]->text);
     std::array<int, 3> o = {0, 0, 0};
-    bool reg = k.fast && g.dims == 3;
+    bool reg = k.fast && g.dims == (k.rows2d ? 2 : 3);
     for (int a = 0; reg && a < g.dims; ++a)
       reg = RegularIndex(k, *e->kids[2 + a], a, &o[a]);
+    if (k.rows2d) o = {o[0], 0, o[1]};  // the rows view's (dx, 0, dy)
     bool zero = o[0] == 0 && o[1] == 0 && o[2] == 0;
     if (reg && g.mode == GridParam::kStaged && (zero || getp == g.periodic)) {
       for (size_t i = 0; i < g.offsets.size(); ++i)
@@ -1442,6 +1460,8 @@ std::string ViewArg(const KernelInfo &k, const GridParam &g) {
            ", " + std::to_string(g.member_index) + ", " +
            std::to_string(g.member_comp) + ")";
   if (g.user) return "__PSUdtViewOf_" + g.uname + "(s." + g.name + ")";
+  if (k.rows2d && k.fast && g.dims == 2)
+    return "physis_hip::RowView<" + g.dev_view + ">(s." + g.name + ")";
   return "physis_hip::DevView<" + g.dev_view + ">(s." + g.name + ")";
 }
 
@@ -1511,7 +1531,10 @@ std::string Translator::EmitStencilStruct(const KernelInfo &k) {
 std::string Translator::EmitContext(const KernelInfo &k) {
   std::ostringstream o;
   o << "// Stencil analysis of " << k.name << " ("
-    << (k.fast ? "2.5D run kernel" : "generic run kernel: " + k.slow_reason)
+    << (!k.fast    ? "generic run kernel: " + k.slow_reason
+        : k.rows2d ? std::string("2.5D run kernel over the rows view nx x 1 x "
+                                 "ny of its 2-D grids, y marched as z")
+                   : std::string("2.5D run kernel"))
     << "):\n";
   for (auto &g : k.grids) {
     o << "//   " << g.name << "  ";
@@ -1707,6 +1730,11 @@ std::string Translator::EmitFastRunKernel(const KernelInfo &k) {
       continue;
     else if (g.user)
       o << "    ctx." << g.name << "_u = " << g.name << ";\n";
+    else if (k.rows2d)  // the stencil sees the 2-D extents (nx, ny)
+    o << "    ctx." << g.name << "_p = " << g.name << ".p;\n"
+      << "    ctx." << g.name << "_dim[0] = " << g.name << ".dim[0];\n"
+      << "    ctx." << g.name << "_dim[1] = " << g.name << ".dim[2];\n"
+      << "    ctx." << g.name << "_dim[2] = 1;\n";
     else
     o << "    ctx." << g.name << "_p = " << g.name << ".p;\n"
       << "    for (int d = 0; d < 3; ++d) ctx." << g.name << "_dim[d] = "
@@ -1798,6 +1826,7 @@ std::string Translator::EmitFastRunKernel(const KernelInfo &k) {
     if (g.emitted) o << "              ctx." << g.name << "_e = false;\n";
   }
   std::vector<std::string> args = {"x", "yrow", "z", "ctx"};
+  if (k.rows2d) args = {"x", "z", "ctx"};  // the rows view marches y as z
   for (auto &s : k.scalars) args.push_back(s.name);
   if (red)
     o << "              const " << RT << " __r = " << n << "(" << Join(args, ", ")
@@ -1948,11 +1977,13 @@ std::string Translator::EmitGenericRunKernel(const KernelInfo &k) {
 std::string Translator::InteriorCond(const KernelInfo &k, const std::string &y,
                                      const std::string &z) const {
   std::vector<std::string> c;
-  const std::string *iv[3] = {nullptr, &y, &z};
-  for (int a = 1; a <= 2; ++a) {
+  // (a 2-D kernel in the rows view: its y is the kernel's z)
+  const std::string *iv[3] = {nullptr, k.rows2d ? &z : &y, k.rows2d ? nullptr : &z};
+  const int nd = k.rows2d ? 2 : 3;
+  for (int a = 1; a < nd; ++a) {
     if (!k.compares_axis[a]) continue;
     for (auto &g : k.grids) {
-      if (g.user || g.dims != 3 || !(g.read || g.emitted)) continue;
+      if (g.user || g.dims != nd || !(g.read || g.emitted)) continue;
       c.push_back(*iv[a] + " >= 1 && " + *iv[a] + " <= ctx." + g.name + "_dim[" +
                   std::to_string(a) + "] - 2");
     }
@@ -1963,11 +1994,12 @@ std::string Translator::InteriorCond(const KernelInfo &k, const std::string &y,
 std::string Translator::InteriorAssume(const KernelInfo &k, const std::string &y,
                                        const std::string &z) const {
   std::string s;
-  const std::string *iv[3] = {nullptr, &y, &z};
-  for (int a = 1; a <= 2; ++a) {
+  const std::string *iv[3] = {nullptr, k.rows2d ? &z : &y, k.rows2d ? nullptr : &z};
+  const int nd = k.rows2d ? 2 : 3;
+  for (int a = 1; a < nd; ++a) {
     if (!k.compares_axis[a]) continue;
     for (auto &g : k.grids) {
-      if (g.user || g.dims != 3 || !(g.read || g.emitted)) continue;
+      if (g.user || g.dims != nd || !(g.read || g.emitted)) continue;
       const std::string d = "ctx." + g.name + "_dim[" + std::to_string(a) + "]";
       s += "__builtin_assume(" + *iv[a] + " >= 1); __builtin_assume(" + *iv[a] +
            " <= " + d + " - 2); __builtin_assume(" + *iv[a] +
@@ -2230,14 +2262,17 @@ std::string Translator::EmitLauncher(const KernelInfo &k) {
   const std::string &n = k.name;
   const bool red = k.ret_type != "void";
   const std::string RT = red ? k.ret_type : "float";
-  std::vector<std::string> args = {"s.dom"};
+  // a 2-D kernel on the 2.5D path runs over the rows view of its domain
+  const std::string D = k.fast && k.rows2d ? "physis_hip::RowDomain(s.dom)" : "s.dom";
+  std::vector<std::string> args = {D};
   for (auto &g : k.grids)
     if (!g.split) args.push_back(ViewArg(k, g));
   for (auto &s : k.scalars) args.push_back("s." + s.name);
   std::string A = Join(args, ", ");
   if (k.fast) {
-    // LDS budget picks the block height.
-    int by = cfg_.block_waves;
+    // LDS budget picks the block height (one wave for the rows view, whose
+    // y extent is one row).
+    int by = k.rows2d ? 1 : cfg_.block_waves;
     if (by == 0) {
       // A 1024-thread block allows 128 VGPRs per lane, a 512-thread one 256.
       const int est = RegEstimate(k, PrefetchFor(k));
@@ -2274,9 +2309,10 @@ std::string Translator::EmitLauncher(const KernelInfo &k) {
       << "  static int per_cu = physis_hip::BlocksPerCU(\n"
       << "      (const void *)__PSStencilRun_" << n << "<VX, " << by
       << ", -1>, 64 * " << by << ");\n"
-      << "  return physis_hip::PlanRun(s.dom, 64 * VX, " << by << " - 2 * "
-      << k.hy << ", per_cu, physis_hip::WindowChunkLimit({" << Join(wgrids, ", ")
-      << "}, " << margin << "));\n}\n\n";
+      << "  return physis_hip::PlanRun(" << D << ", 64 * VX, " << by << " - 2 * "
+      << k.hy << ", per_cu, physis_hip::WindowChunkLimit"
+      << (k.rows2d ? "Rows" : "") << "({" << Join(wgrids, ", ") << "}, "
+      << margin << "));\n}\n\n";
     // red-black maps: one launch per colour, red first
     const std::string RBA = k.used_rb ? ", __c" : "";
     o << "template <int VX>\nstatic int __PSLaunch_" << n << "(const struct __PSStencil_"

@@ -80,6 +80,10 @@ struct KernelInfo {
   std::vector<std::pair<char, int>> order;
   std::string ret_type = "void";
   bool fast = false;          // 2.5D template path
+  // A 2-D kernel on the 2.5D path: its grids are viewed as nx x 1 x ny 3-D
+  // grids (the same linear offsets), so the kernel marches y as z with
+  // one-wave blocks; offsets are stored as (dx, 0, dy).
+  bool rows2d = false;
   // an index parameter is redeclared, assigned, incremented or has its
   // address taken in the body: every get is treated as irregular
   bool index_rebound = false;

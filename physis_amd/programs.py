@@ -92,6 +92,26 @@ class Udt7:
         self.lib.udt7_run(g, count, *[float(c) for c in coef])
 
 
+class Stencil2D:
+    """physis_amd/programs/stencil2d.c: 2-D 5- and 9-point maps over the
+    interior of 2-D float grids."""
+
+    def __init__(self):
+        L = rt.load_program("stencil2d")
+        G = rt.PSGridPtr
+        L.stencil2d_run.argtypes = [G, G, C.c_int, C.c_int, _F]
+        L.stencil2d_sweep.argtypes = [G, G, C.c_int, _F]
+        self.lib = L
+
+    def run(self, a, b, points: int, count: int, w: float) -> None:
+        """count pairs of sweeps a -> b -> a (points = 5 or 9)."""
+        self.lib.stencil2d_run(a.handle, b.handle, points, count, float(w))
+
+    def sweep(self, a, b, points: int, w: float) -> None:
+        """One sweep a -> b."""
+        self.lib.stencil2d_sweep(a.handle, b.handle, points, float(w))
+
+
 class Himeno:
     """physis_amd/programs/himeno.c: 19-point Himeno Jacobi sweeps."""
 

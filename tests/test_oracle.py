@@ -361,3 +361,38 @@ def test_udt7_oracle_matches_reference_output():
     dump = "".join("%f %f\n" % (a, b) for a, b in rows.tolist()).encode()
     assert sha256(dump) == \
         GOLD["system_tests"]["test_user-defined-type-7-pt"]["sha256"]
+
+
+@pytest.mark.parametrize("points", [5, 9])
+def test_stencil2d_oracle_matches_reference_output(points):
+    """oracle_stencil2d_run with w = 1 and one sweep is the reference's
+    test_5-pt-2d / test_9-pt-2d (input i, output grid initialised to the
+    input, one map over [1, N-1)^2; integer sums below 2^24 are exact): its
+    "%f" dump hashes to the reference oracle's output."""
+    n = 32
+    a = np.arange(n * n, dtype=np.float32)
+    b = a.copy()
+    lib().oracle_stencil2d_run(a, b, n, n, points, 1.0, 1)
+    dump = "".join("%f\n" % v for v in b.astype(np.float64).tolist()).encode()
+    assert sha256(dump) == \
+        GOLD["system_tests"][f"test_{points}-pt-2d"]["sha256"]
+
+
+@pytest.mark.parametrize("points", [5, 9])
+def test_stencil2d_oracle_linear_field(points):
+    """A linear field x + 1000 y on a non-square grid: every interior point
+    sums to points * f exactly (the neighbour offsets cancel), points outside
+    keep their value, and a second sweep maps back into a; a transposed
+    index or a wrong neighbour breaks it."""
+    nx, ny = 37, 23
+    f = (np.arange(nx)[None, :] + 1000.0 * np.arange(ny)[:, None]).astype(
+        np.float32).reshape(-1)
+    a, b = f.copy(), np.full_like(f, -1.0)
+    lib().oracle_stencil2d_run(a, b, nx, ny, points, 1.0, 1)
+    want = np.full((ny, nx), -1.0, np.float32)
+    want[1:-1, 1:-1] = points * f.reshape(ny, nx)[1:-1, 1:-1]
+    assert np.array_equal(b.reshape(ny, nx), want)
+    lib().oracle_stencil2d_run(a, b, nx, ny, points, 0.5, 2)
+    assert np.array_equal(b.reshape(ny, nx)[1:-1, 1:-1],
+                          (points * f.reshape(ny, nx)[1:-1, 1:-1] *
+                           np.float32(0.5)).astype(np.float32))

@@ -10,7 +10,8 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PHYSISC = os.path.join(ROOT, "physis_amd", "bin", "physisc-hip")
 SOURCES = [os.path.join(ROOT, "physis_amd", "programs", f)
-           for f in ("diffusion7.c", "himeno.c", "redblack7.c", "udt7.c")] + \
+           for f in ("diffusion7.c", "himeno.c", "redblack7.c", "udt7.c",
+                     "stencil2d.c")] + \
           [os.path.join(ROOT, "tests", "programs", "systests.c")]
 
 
@@ -62,19 +63,49 @@ def test_irregular_get_reads_memory():
     assert "x[-3,3]" in code and "periodic" in code
 
 
-def test_two_dimensional_kernel_uses_generic_path():
+def test_two_dimensional_kernel_runs_in_the_rows_view():
+    """A 2-D map over 2-D grids runs on the 2.5D kernel with its grids
+    viewed as nx x 1 x ny (y marched as z): offsets (dx, dy) become
+    (dx, 0, dy), the stencil sees its 2-D indices and extents, and the
+    blocks are one wave.  A 2-D map reading a 1-D grid, and a 1-D map, keep
+    the generic kernel."""
     src = """#include "physis/physis.h"
 static void k(const int x, const int y, PSGrid2DFloat a, PSGrid2DFloat b) {
-  PSGridEmit(b, PSGridGet(a, x+1, y) + PSGridGet(a, x-1, y));
+  float v = PSGridGet(a, x+1, y) + PSGridGet(a, x-1, y - 1);
+  if (y < PSGridDim(a, 1) - 2) v += PSGridGet(a, x, y + 2);
+  PSGridEmit(b, v);
 }
-void run(PSGrid2DFloat a, PSGrid2DFloat b) {
+static void m(const int x, const int y, PSGrid2DFloat a, PSGrid1DFloat c,
+              PSGrid2DFloat b) {
+  PSGridEmit(b, PSGridGet(a, x, y) * PSGridGet(c, x));
+}
+static void l(const int x, PSGrid1DFloat c, PSGrid1DFloat e) {
+  PSGridEmit(e, PSGridGet(c, x + 1));
+}
+void run(PSGrid2DFloat a, PSGrid2DFloat b, PSGrid1DFloat c, PSGrid1DFloat e) {
   PSDomain2D d = PSDomain2DNew(1, 7, 0, 8);
-  PSStencilRun(PSStencilMap(k, d, a, b));
+  PSDomain1D d1 = PSDomain1DNew(0, 7);
+  PSStencilRun(PSStencilMap(k, d, a, b), PSStencilMap(m, d, a, c, b));
+  PSStencilRun(PSStencilMap(l, d1, c, e));
 }
 """
     rc, code, err = translate(None, src)
     assert rc == 0, err
-    assert "__PSStencilRunGeneric_k" in code
+    assert "Stencil analysis of k (2.5D run kernel over the rows view" in code
+    assert "offsets 0:(1,0,0) 1:(-1,0,-1) 2:(0,0,2)" in code
+    assert "staged x[-1,1] y[-0,0] z[-1,2] non-periodic" in code
+    assert "k(x, z, ctx);" in code
+    assert "ctx.a_dim[1] = a.dim[2];" in code  # PSGridDim(a, 1) is ny
+    assert "__PSStencilRun_k<VX, 1, -1>), grid, block, 0, stream, " \
+        "physis_hip::RowDomain(s.dom), physis_hip::RowView<" \
+        "__PSGrid3DFloat_dev>(s.a)" in code
+    assert "physis_hip::WindowChunkLimitRows({s.a, s.b}, 3)" in code
+    # the y test folds in the interior form: y, the kernel's z, off the
+    # first and last rows of a
+    assert "z >= 1 && z <= ctx.a_dim[1] - 2" in code
+    assert "__PSStencilRunGeneric_m" in code and "__PSStencilRunGeneric_l" in code
+    assert "(generic run kernel: a 2-D stencil over grids that are not all " \
+        "2-D grids of a primitive type)" in code
 
 
 def test_red_black_maps_visit_one_colour():
