@@ -247,6 +247,13 @@ STENCIL_FRAGS = [
     ("one-sweep 7-pt (clamped)", "__PSStencilRun_diffusion7_kernel<"),
     ("two-sweep 7-pt (periodic)", "__PSStencilRun2_diffusion7_periodic_kernel<"),
     ("512 MiB copy", "copyBuffer")]
+# the 19-point kernels of physis_amd.profile_kernels (Himeno 512x256x256,
+# precision tier 512^3)
+POISSON_FRAGS = [
+    ("Himeno two-sweep", "__PSStencilRun2_jacobi_kernel<"),
+    ("BF16 inner two-sweep", "TwoSweepKernel<physis_hip::bf16_t"),
+    ("FP64 update + residual", "ResidualKernel<double, physis_hip::bf16_t, 2, true"),
+    ("512 MiB copy", "copyBuffer")]
 # the level-1 kernels of physis_amd.profile_unet (batch 8 of 128^3)
 UNET_FRAGS = [
     ("forward 8->8", "ConvZPairKernel<8, true"),
@@ -321,6 +328,10 @@ def counters_md(dirs, out_md: str, command: str, frags=None,
 if __name__ == "__main__":
     if len(sys.argv) >= 5 and sys.argv[1] == "--counters":
         counters_md(sys.argv[2:-2], sys.argv[-2], sys.argv[-1])
+        sys.exit(0)
+    if len(sys.argv) >= 5 and sys.argv[1] == "--counters-19pt":
+        counters_md(sys.argv[2:-2], sys.argv[-2], sys.argv[-1], POISSON_FRAGS,
+                    "the 19-point kernels")
         sys.exit(0)
     if len(sys.argv) >= 5 and sys.argv[1] == "--counters-unet":
         counters_md(sys.argv[2:-2], sys.argv[-2], sys.argv[-1], UNET_FRAGS,
