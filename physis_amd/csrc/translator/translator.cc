@@ -1975,15 +1975,17 @@ std::string Translator::EmitGenericRunKernel(const KernelInfo &k) {
 // only loads), and u''(s-2) from ring B on the tile's rows; one barrier per
 // step.
 std::string Translator::InteriorCond(const KernelInfo &k, const std::string &y,
-                                     const std::string &z) const {
+                                     const std::string &z, int only) const {
   std::vector<std::string> c;
   // (a 2-D kernel in the rows view: its y is the kernel's z)
   const std::string *iv[3] = {nullptr, k.rows2d ? &z : &y, k.rows2d ? nullptr : &z};
   const int nd = k.rows2d ? 2 : 3;
   for (int a = 1; a < nd; ++a) {
     if (!k.compares_axis[a]) continue;
-    for (auto &g : k.grids) {
+    for (int gi = 0; gi < (int)k.grids.size(); ++gi) {
+      const GridParam &g = k.grids[gi];
       if (g.user || g.dims != nd || !(g.read || g.emitted)) continue;
+      if (only >= 0 && gi != only) continue;
       c.push_back(*iv[a] + " >= 1 && " + *iv[a] + " <= ctx." + g.name + "_dim[" +
                   std::to_string(a) + "] - 2");
     }
@@ -1992,14 +1994,16 @@ std::string Translator::InteriorCond(const KernelInfo &k, const std::string &y,
 }
 
 std::string Translator::InteriorAssume(const KernelInfo &k, const std::string &y,
-                                       const std::string &z) const {
+                                       const std::string &z, int only) const {
   std::string s;
   const std::string *iv[3] = {nullptr, k.rows2d ? &z : &y, k.rows2d ? nullptr : &z};
   const int nd = k.rows2d ? 2 : 3;
   for (int a = 1; a < nd; ++a) {
     if (!k.compares_axis[a]) continue;
-    for (auto &g : k.grids) {
+    for (int gi = 0; gi < (int)k.grids.size(); ++gi) {
+      const GridParam &g = k.grids[gi];
       if (g.user || g.dims != nd || !(g.read || g.emitted)) continue;
+      if (only >= 0 && gi != only) continue;
       const std::string d = "ctx." + g.name + "_dim[" + std::to_string(a) + "]";
       s += "__builtin_assume(" + *iv[a] + " >= 1); __builtin_assume(" + *iv[a] +
            " <= " + d + " - 2); __builtin_assume(" + *iv[a] +
@@ -2016,13 +2020,13 @@ std::string Translator::InteriorAssume(const KernelInfo &k, const std::string &y
 std::string Translator::SplitInterior(const KernelInfo &k, const std::string &y,
                                       const std::string &z,
                                       const std::string &body,
-                                      const std::string &ind) const {
-  const std::string cond = cfg_.interior_split ? InteriorCond(k, y, z) : "";
+                                      const std::string &ind, int only) const {
+  const std::string cond = cfg_.interior_split ? InteriorCond(k, y, z, only) : "";
   if (cond.empty()) return body;
   std::ostringstream o;
   o << ind << "auto __pts = [&](auto __interior) {\n"
     << ind << "  if constexpr (decltype(__interior)::value) {\n"
-    << ind << "    " << InteriorAssume(k, y, z) << "\n" << ind << "  }\n"
+    << ind << "    " << InteriorAssume(k, y, z, only) << "\n" << ind << "  }\n"
     << body << ind << "};\n"
     << ind << "if (" << cond << ")\n"
     << ind << "  __pts(physis_hip::Flag<true>{});\n"
@@ -2056,34 +2060,38 @@ std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
     }
     return f.str();
   };
+  // Row arrays of the offsets' rows from a ring of six slots: the planes
+  // at dz = -1, 0, 1 of the centre are in slots sm, sc, sp.
+  auto slot = [](int dz) { return dz < 0 ? "sm" : (dz > 0 ? "sp" : "sc"); };
   auto read_rows = [&](const std::string &pre, const std::string &lds,
-                       const std::string &S, const std::string &base,
-                       const std::string &ind) {
+                       const std::string &S, const std::string &ind) {
     std::ostringstream f;
     for (size_t r = 0; r < rows.size(); ++r) {
       const Row &w = rows[r];
       f << ind << T << " " << pre << "_r" << r << "[VX + " << w.xr - w.xl
         << "];\n";
       f << ind << S << "::template ReadRow<" << w.xl << ", " << w.xr << ">("
-        << lds << " + physis_hip::RingSlot<4>(" << base << ", " << w.dz + 1
-        << ") * " << S << "::kPlane, wave + " << w.dy << ", lane, " << pre
-        << "_r" << r << ");\n";
+        << lds << " + " << slot(w.dz) << " * " << S << "::kPlane, wave + "
+        << w.dy << ", lane, " << pre << "_r" << r << ");\n";
     }
     return f.str();
   };
   o << "// Two-sweep run kernel of " << n
     << " (temporal blocking): one pass over src\n"
        "// computes u' = " << n << "(src) on the tile and a one-point halo and\n"
-       "// u'' = " << n << "(u') two planes behind; dst <- u'', mid <- u' when\n"
-       "// mid is not NULL.\n";
+       "// u'' = " << n << "(u') three planes behind; dst <- u'', mid <- u' when\n"
+       "// mid is not NULL.  Two planes per barrier: an interval computes u' of\n"
+       "// planes s0 and s0 + 1 and u'' of s0 - 3 and s0 - 2 from rings of six\n"
+       "// slots (source plane p in slot (p - zb + 2) % 6, u'(p) in slot\n"
+       "// (p - zb + 1) % 6), then stores source planes s0 + 3 and s0 + 4.\n";
   o << "template <int VX, int BY>\n__global__ __launch_bounds__(64 * BY) void __PSStencilRun2_"
     << n << "(\n    __PSDomain dom, " << V << " src, " << V << " dst, " << T
     << " *__restrict__ mid" << (sp.empty() ? "" : ", " + Join(sp, ", "))
     << ", physis_hip::RunShape sh) {\n";
   o << "  constexpr int PD = 2, TX = 64 * VX, TY = BY - 4;\n"
-    << "  typedef physis_hip::Stage<" << T << ", VX, BY, 2, 2, 4, " << per
+    << "  typedef physis_hip::Stage<" << T << ", VX, BY, 2, 2, 6, " << per
     << "> SA;\n"
-    << "  typedef physis_hip::Stage<" << T << ", VX, BY, 1, 1, 4, false> SB;\n"
+    << "  typedef physis_hip::Stage<" << T << ", VX, BY, 1, 1, 6, false> SB;\n"
     << "  __shared__ __attribute__((aligned(16))) " << T << " a_lds[SA::kLdsElems];\n"
     << "  __shared__ __attribute__((aligned(16))) " << T << " b_lds[SB::kLdsElems];\n"
     << "  const int lane = threadIdx.x % 64;\n"
@@ -2111,87 +2119,99 @@ std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
     << ">::Make(mid ? mid : dst.p, n, (long)zb * nxy, (long)ze * nxy);\n"
     << "  SA st[PD];\n"
        "  const typename SA::Addr ra = SA::Prepare(src.dim, x0, yrow, lane);\n"
-       "#pragma unroll\n  for (int k = 0; k < 3; ++k) {\n"
+       "#pragma unroll\n  for (int k = 0; k < 4; ++k) {\n"
        "    st[0].Load(aw, src.p, src.dim, ra, zb - 2 + k);\n"
        "    st[0].Store(a_lds + k * SA::kPlane, wave, lane);\n  }\n"
        "#pragma unroll\n  for (int k = 0; k < PD; ++k)\n"
-       "    st[k].Load(aw, src.p, src.dim, ra, zb + 1 + k);\n"
-
+       "    st[k].Load(aw, src.p, src.dim, ra, zb + 2 + k);\n"
        "  __syncthreads();\n"
     << "  __PSCtx_" << n << " ctx;\n"
     << "  ctx." << I << "_p = src.p;\n  ctx." << O << "_p = dst.p;\n"
+    // (a fusable pair's grids have equal extents: one set of extents lets
+    // the compiler fold the interior tests of both grids into one)
     << "  for (int d = 0; d < 3; ++d) {\n    ctx." << I << "_dim[d] = src.dim[d];\n"
-    << "    ctx." << O << "_dim[d] = dst.dim[d];\n  }\n"
-    // four steps per iteration: both rings' oldest planes are in slot k
-    // at step k, so slots are compile-time indices
-    << "  for (int sc = zb - 1; sc <= ze + 1; sc += 4) {\n#pragma unroll\n"
-       "    for (int k = 0; k < 4; ++k) {\n"
-       "      const int s = sc + k;\n"
-       "      const int abase = k, bbase = k;\n"
-       "      if (s > ze + 1) break;  // uniform over the block\n"
-       "      if (mid_row && s <= ze) {\n";
-  o << read_rows("a", "a_lds", "SA", "abase", "        ");
-  o << "        " << T << " uo[VX];\n";
+    << "    ctx." << O << "_dim[d] = src.dim[d];\n  }\n";
+  // u'(s) of the wave's row, source planes s - 1, s, s + 1 in A slots sm,
+  // sc, sp, into B slot bo; the halo columns x0 - 1 and x0 + TX of u' rows
+  // 1 .. BY-2 by wave 0 (which otherwise only loads), lane 2(r-1) + c
+  o << "  auto __uprime = [&](const int s, const int sm, const int sc, "
+       "const int sp, const int bo) {\n"
+       "    if (mid_row && s <= ze) {\n";
+  o << read_rows("a", "a_lds", "SA", "      ");
+  o << "      " << T << " uo[VX];\n";
   {
     std::ostringstream b;
-    b << "#pragma unroll\n        for (int i = 0; i < VX; ++i) {\n";
-    b << fill("a", "          ");
-    b << "          ctx." << O << "_e = false;\n"
-      << "          " << n << "(xs + i, yrow, s, " << fargs << ");\n"
-      << "          uo[i] = ctx." << O << "_v;\n        }\n";
-    o << SplitInterior(k, "yrow", "s", b.str(), "        ");
+    b << "#pragma unroll\n      for (int i = 0; i < VX; ++i) {\n";
+    b << fill("a", "        ");
+    b << "        ctx." << O << "_e = false;\n"
+      << "        " << n << "(xs + i, yrow, s, " << fargs << ");\n"
+      << "        uo[i] = ctx." << O << "_v;\n      }\n";
+    o << SplitInterior(k, "yrow", "s", b.str(), "      ", k.fuse_in);
   }
-  o << "        SB::WriteRow(b_lds + physis_hip::RingSlot<4>(bbase, 3) * "
-       "SB::kPlane, wave, lane, uo);\n";
-  o << "        if (mid && out_row && s >= zb && s < ze) {\n"
-       "          unsigned m = 0;\n"
-       "#pragma unroll\n          for (int i = 0; i < VX; ++i)\n"
-       "            if (xs + i >= dom.local_min[0] && xs + i < dom.local_max[0]) m |= 1u << i;\n"
-    << "          physis_hip::StoreMasked<" << T
-    << ", VX>(mw, xs + yrow * nx + s * nxy, uo, m);\n        }\n      }\n";
-  // the halo columns x0 - 1 and x0 + TX of u' rows 1 .. BY-2, one point
-  // per lane of wave 0 (which loads only): lane 2(r-1) + c, column c
-  o << "      if (wave == 0 && lane < 2 * (BY - 2) && s <= ze) {\n"
-       "        const int hr = 1 + (lane >> 1);\n"
-       "        const int px = (lane & 1) ? TX : -1;\n";
+  o << "      SB::WriteRow(b_lds + bo * SB::kPlane, wave, lane, uo);\n";
+  o << "      if (mid && out_row && s >= zb && s < ze) {\n"
+       "        unsigned m = 0;\n"
+       "#pragma unroll\n        for (int i = 0; i < VX; ++i)\n"
+       "          if (xs + i >= dom.local_min[0] && xs + i < dom.local_max[0]) m |= 1u << i;\n"
+    << "        physis_hip::StoreMasked<" << T
+    << ", VX>(mw, xs + yrow * nx + s * nxy, uo, m);\n      }\n    }\n";
+  o << "    if (wave == 0 && lane < 2 * (BY - 2) && s <= ze) {\n"
+       "      const int hr = 1 + (lane >> 1);\n"
+       "      const int px = (lane & 1) ? TX : -1;\n";
   for (size_t i = 0; i < gi.offsets.size(); ++i) {
     const auto &off = gi.offsets[i];
-    o << "        ctx." << I << "[" << i << "] = a_lds[physis_hip::RingSlot<4>(abase, "
-      << off[2] + 1 << ") * SA::kPlane + (hr + " << off[1]
+    o << "      ctx." << I << "[" << i << "] = a_lds[" << slot(off[2])
+      << " * SA::kPlane + (hr + " << off[1]
       << ") * SA::kRow + SA::kPad + px + " << off[0] << "];\n";
   }
-  o << "        ctx." << O << "_e = false;\n"
-    << "        " << n << "(x0 + px, y0 - 2 + hr, s, " << fargs << ");\n"
-    << "        b_lds[physis_hip::RingSlot<4>(bbase, 3) * SB::kPlane + hr * SB::kRow "
-       "+ SB::kPad + px] = ctx." << O << "_v;\n      }\n";
-  o << "      if (out_row && s - 2 >= zb) {\n";
-  o << read_rows("b", "b_lds", "SB", "bbase", "        ");
+  o << "      ctx." << O << "_e = false;\n"
+    << "      " << n << "(x0 + px, y0 - 2 + hr, s, " << fargs << ");\n"
+    << "      b_lds[bo * SB::kPlane + hr * SB::kRow + SB::kPad + px] = ctx."
+    << O << "_v;\n    }\n  };\n";
+  // u''(z) of the wave's row, u'(z - 1), u'(z), u'(z + 1) in B slots sm,
+  // sc, sp
+  o << "  auto __udouble = [&](const int z, const int sm, const int sc, "
+       "const int sp) {\n"
+       "    if (out_row && z >= zb && z < ze) {\n";
+  o << read_rows("b", "b_lds", "SB", "      ");
   // every point of the row is computed (the values come from LDS); the
   // store mask keeps the grid's columns
-  o << "        " << T << " uu[VX];\n        unsigned m = 0;\n";
+  o << "      " << T << " uu[VX];\n      unsigned m = 0;\n";
   {
     std::ostringstream b;
-    b << "#pragma unroll\n        for (int i = 0; i < VX; ++i) {\n";
-    b << fill("b", "          ");
-    b << "          ctx." << O << "_e = false;\n"
-      << "          " << n << "(xs + i, yrow, s - 2, " << fargs << ");\n"
-      << "          uu[i] = ctx." << O << "_v;\n"
-      << "          m |= (xs + i >= dom.local_min[0] && xs + i < dom.local_max[0]) "
-         "? 1u << i : 0u;\n        }\n";
-    o << SplitInterior(k, "yrow", "s - 2", b.str(), "        ");
+    b << "#pragma unroll\n      for (int i = 0; i < VX; ++i) {\n";
+    b << fill("b", "        ");
+    b << "        ctx." << O << "_e = false;\n"
+      << "        " << n << "(xs + i, yrow, z, " << fargs << ");\n"
+      << "        uu[i] = ctx." << O << "_v;\n"
+      << "        m |= (xs + i >= dom.local_min[0] && xs + i < dom.local_max[0]) "
+         "? 1u << i : 0u;\n      }\n";
+    o << SplitInterior(k, "yrow", "z", b.str(), "      ", k.fuse_in);
   }
-  o << "        physis_hip::StoreMasked<" << T
-    << ", VX>(dw, xs + yrow * nx + (s - 2) * nxy, uu, m);\n      }\n"
-    << "      st[k % PD].Store(a_lds + physis_hip::RingSlot<4>(abase, 3) * SA::kPlane, "
-       "wave, lane);\n"
+  o << "      physis_hip::StoreMasked<" << T
+    << ", VX>(dw, xs + yrow * nx + z * nxy, uu, m);\n    }\n  };\n";
+  // three intervals per iteration, so that ring slots are compile-time
+  o << "  for (int t3 = 0;; t3 += 3) {\n#pragma unroll\n"
+       "    for (int j = 0; j < 3; ++j) {\n"
+       "      const int s0 = zb - 1 + 2 * (t3 + j);\n"
+       "      if (s0 - 3 > ze - 1) return;  // uniform over the block: every u'' done\n"
+       "      __uprime(s0, (2 * j) % 6, (2 * j + 1) % 6, (2 * j + 2) % 6, (2 * j) % 6);\n"
+       "      __uprime(s0 + 1, (2 * j + 1) % 6, (2 * j + 2) % 6, (2 * j + 3) % 6, "
+       "(2 * j + 1) % 6);\n"
+       "      __udouble(s0 - 3, (2 * j + 2) % 6, (2 * j + 3) % 6, (2 * j + 4) % 6);\n"
+       "      __udouble(s0 - 2, (2 * j + 3) % 6, (2 * j + 4) % 6, (2 * j + 5) % 6);\n"
+       "      st[0].Store(a_lds + ((2 * j + 4) % 6) * SA::kPlane, wave, lane);\n"
+       "      st[1].Store(a_lds + ((2 * j + 5) % 6) * SA::kPlane, wave, lane);\n"
        "      __syncthreads();\n"
-       "      st[k % PD].Load(aw, src.p, src.dim, ra, s + 2 + PD);\n"
+       "      st[0].Load(aw, src.p, src.dim, ra, s0 + 5);\n"
+       "      st[1].Load(aw, src.p, src.dim, ra, s0 + 6);\n"
        "    }\n  }\n}\n\n";
-  // launcher.  Launch shape: 4 points per lane in 16-wave blocks (12
-  // output rows, one block per CU by LDS); measured on the 512^3 clamped
-  // kernel in one run: 258 us per launch, against 294 in 12-wave blocks
-  // and 301 / 373 / 431 us at 2 points per lane in 12 / 8 / 16-wave blocks.
-  const int vx = 4, by = 16;
+  // launcher.  Launch shape: 4 points per lane in 12-wave blocks (8
+  // output rows; the two six-slot rings take 152 KB of LDS, one block per
+  // CU); measured on the 512^3 clamped kernel: 239 us per launch, against
+  // 374 us in 8-wave blocks, and 257 us for one plane per barrier in
+  // 16-wave blocks with four-slot rings.
+  const int vx = 4, by = 12;
   o << "static constexpr int __PS2VX_" << n << " = " << vx << ", __PS2BY_"
     << n << " = " << by << ";\n";
   std::vector<std::string> largs = {"s.dom", "src", "dst", "mid"};

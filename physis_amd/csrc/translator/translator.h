@@ -158,14 +158,16 @@ class Translator {
   // Interior split of a wave step at rows y, plane z: the wave-uniform
   // condition and the facts assumed under it ("" when the body compares
   // neither y nor z).
+  // only >= 0: the extents of grid k.grids[only] stand for every grid's
+  // (two-sweep passes, whose grids have equal extents).
   std::string InteriorCond(const KernelInfo &k, const std::string &y,
-                           const std::string &z) const;
+                           const std::string &z, int only = -1) const;
   std::string InteriorAssume(const KernelInfo &k, const std::string &y,
-                             const std::string &z) const;
+                             const std::string &z, int only = -1) const;
   // the lambda-wrapped element loop `body` called in its two forms
   std::string SplitInterior(const KernelInfo &k, const std::string &y,
                             const std::string &z, const std::string &body,
-                            const std::string &ind) const;
+                            const std::string &ind, int only = -1) const;
   // VGPRs per lane of the run kernel at VX = 4 with pd staged planes in
   // flight (a rough estimate), and the prefetch depth used for k.
   int RegEstimate(const KernelInfo &k, int pd) const;
