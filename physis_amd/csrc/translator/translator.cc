@@ -1123,8 +1123,12 @@ void Translator::Decide(KernelInfo &k) {
   }
   for (auto &g : k.grids)
     if (k.fast && g.dims != (k.rows2d ? 2 : 3) && (g.read || g.emitted)) {
+      // a 3-D map may read lower-dimensional grids (coefficients along an
+      // axis: the reference's test_mixed-dim*); their gets read memory
+      if (!k.rows2d && g.dims < 3 && !g.emitted && !g.user) continue;
       k.fast = false;
-      k.slow_reason = "grid " + g.name + " is not 3-D";
+      k.slow_reason = g.emitted ? "grid " + g.name + " is emitted and not 3-D"
+                                : "grid " + g.name + " is not 3-D";
     }
   if (!k.fast) return;
   if (k.rows2d)
@@ -1730,6 +1734,10 @@ std::string Translator::EmitFastRunKernel(const KernelInfo &k) {
       continue;
     else if (g.user)
       o << "    ctx." << g.name << "_u = " << g.name << ";\n";
+    else if (!k.rows2d && g.dims < 3)  // a lower-dimensional grid
+    o << "    ctx." << g.name << "_p = " << g.name << ".p;\n"
+      << "    for (int d = 0; d < 3; ++d) ctx." << g.name << "_dim[d] = d < "
+      << g.dims << " ? " << g.name << ".dim[d] : 1;\n";
     else if (k.rows2d)  // the stencil sees the 2-D extents (nx, ny)
     o << "    ctx." << g.name << "_p = " << g.name << ".p;\n"
       << "    ctx." << g.name << "_dim[0] = " << g.name << ".dim[0];\n"
@@ -1744,7 +1752,7 @@ std::string Translator::EmitFastRunKernel(const KernelInfo &k) {
   o << "    if constexpr (VX > 1) {\n"
        "      __builtin_assume((xs & (VX - 1)) == 0);\n";
   for (auto &g : k.grids)
-    if (!g.user && (g.read || g.emitted))
+    if (!g.user && (g.read || g.emitted) && g.dims == (k.rows2d ? 2 : 3))
       o << "      __builtin_assume((ctx." << g.name << "_dim[0] & 3) == 0);\n";
   o << "    }\n";
   // The z loop is unrolled over the prefetch depth.  When that is a
@@ -2319,6 +2327,7 @@ std::string Translator::EmitLauncher(const KernelInfo &k) {
     std::vector<std::string> wgrids;
     for (auto &g : k.grids) {
       if (!(g.read || g.emitted) || g.user) continue;
+      if (g.member_of < 0 && g.dims != (k.rows2d ? 2 : 3)) continue;  // not windowed
       if (std::find(wgrids.begin(), wgrids.end(), HostGrid(k, g)) == wgrids.end())
         wgrids.push_back(HostGrid(k, g));
       if (g.mode == GridParam::kStaged)
@@ -2366,7 +2375,9 @@ std::string Translator::EmitLauncher(const KernelInfo &k) {
     };
     for (auto &g : k.grids) {
       if (g.read || g.emitted) {
-        add(conds, HostGrid(k, g) + "->dim[0] % 4 == 0");
+        // (lower-dimensional grids are read through bounds-checked gets)
+        if (g.member_of >= 0 || g.dims == (k.rows2d ? 2 : 3))
+          add(conds, HostGrid(k, g) + "->dim[0] % 4 == 0");
         add(checks, g.member_of >= 0
                         ? "physis_hip::CheckUserGrid3D(" + HostGrid(k, g) + ")"
                         : "physis_hip::CheckGrid3D(s." + g.name + ", " +

@@ -108,6 +108,42 @@ void run(PSGrid2DFloat a, PSGrid2DFloat b, PSGrid1DFloat c, PSGrid1DFloat e) {
         "2-D grids of a primitive type)" in code
 
 
+def test_three_dimensional_map_reading_lower_dimensional_grids():
+    """A 3-D map that reads 1-D and 2-D coefficient grids (the reference's
+    test_mixed-dim3 pattern) runs on the 2.5D kernel: the 3-D grid is
+    staged, the others are read through bounds-checked gets with their own
+    extents, and only 3-D grids decide the vector width and the windows; a
+    map emitting a 2-D grid keeps the generic kernel."""
+    src = """#include "physis/physis.h"
+static void k(const int x, const int y, const int z, PSGrid3DFloat g,
+              PSGrid3DFloat g2, PSGrid1DFloat i, PSGrid2DFloat j) {
+  PSGridEmit(g2, PSGridGet(g, x - 1, y, z) * PSGridGet(i, x - 1) +
+                 PSGridGet(g, x, y, z + 1) * PSGridGet(j, y, z + 1));
+}
+static void m(const int x, const int y, const int z, PSGrid3DFloat g,
+              PSGrid2DFloat j) {
+  PSGridEmit(j, PSGridGet(g, x, y, z));
+}
+void run(PSGrid3DFloat g, PSGrid3DFloat g2, PSGrid1DFloat i, PSGrid2DFloat j) {
+  PSDomain3D d = PSDomain3DNew(1, 7, 1, 7, 1, 7);
+  PSStencilRun(PSStencilMap(k, d, g, g2, i, j), PSStencilMap(m, d, g, j));
+}
+"""
+    rc, code, err = translate(None, src)
+    assert rc == 0, err
+    assert "Stencil analysis of k (2.5D run kernel)" in code
+    assert "i  read from memory, irregular gets" in code
+    assert "physis_hip::GetLinear<float>(__ctx.j_p, __ctx.j_dim, 2, y, z + 1)" in code
+    assert "ctx.i_dim[d] = d < 1 ? i.dim[d] : 1;" in code
+    assert "ctx.j_dim[d] = d < 2 ? j.dim[d] : 1;" in code
+    assert "physis_hip::WindowChunkLimit({s.g, s.g2}, 2)" in code
+    assert "(s.g->dim[0] % 4 == 0 && s.g2->dim[0] % 4 == 0) ?" in code
+    assert "__builtin_assume((ctx.i_dim[0] & 3) == 0)" not in code
+    assert "physis_hip::DevView<__PSGrid1DFloat_dev>(s.i)" in code
+    assert ("Stencil analysis of m (generic run kernel: grid j is emitted and "
+            "not 3-D)") in code
+
+
 def test_red_black_maps_visit_one_colour():
     src = """#include "physis/physis.h"
 static void k(const int x, const int y, const int z, PSGrid3DFloat g) {
