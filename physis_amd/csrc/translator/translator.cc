@@ -560,6 +560,19 @@ struct UserSplit {
       for (size_t i = 2; i < get->kids.size(); ++i) Check(*get->kids[i]);
       return;
     }
+    if (e.kind == EK::kIndex) {
+      // an index chain over a member of a get of g that names no plane (a
+      // runtime component, a scalar member indexed): not split
+      const Expr *b = Unp(&e);
+      while (b && b->kind == EK::kIndex) b = Unp(b->kids[0].get());
+      if (b && b->kind == EK::kMember) {
+        const Expr *call = GetOf(b->kids[0].get());
+        if (call && IsG(call->kids[1].get())) {
+          ok = false;
+          return;
+        }
+      }
+    }
     if (const std::string *c = CalleeName(e)) {
       if (*c == "PSGridEmitUtype" && e.kids.size() == 3 &&
           EmitPlane(e.kids[1].get(), &m, &comp)) {

@@ -324,6 +324,15 @@ def test_user_point_grids_split_into_member_grids():
             "user-defined points)") in code
 
 
+def test_user_point_scalar_member_indexed_is_not_split():
+    """An index applied to a scalar member of a get is not a plane: the
+    grid is not split, and the generic lowering reports the error."""
+    bad = SPLIT_PROG.replace("PSGridGet(g, x, y, z + 1).tag",
+                             "PSGridGet(g, x, y, z + 1).tag[0]")
+    rc, _, err = translate(None, bad)
+    assert rc != 0 and "tag" in err
+
+
 def test_user_point_member_must_be_indexed_to_a_scalar():
     bad = UDT_PROG.replace("PSGridEmitUtype(g.w, (double)s);",
                            "PSGridEmitUtype(g.m[0], s);")
