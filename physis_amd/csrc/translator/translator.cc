@@ -1987,14 +1987,6 @@ std::string Translator::EmitGenericRunKernel(const KernelInfo &k) {
   return o.str();
 }
 
-// Two-sweep run kernel (temporal blocking) of a fusable kernel, its launcher
-// and the run-time check that a map pair may use it.  Block: BY waves, wave
-// r holds source row y0 - 2 + r; ring A (LDS) holds source planes s-1..s+1
-// with a two-point x halo, ring B the intermediate planes s-3..s-1 with a
-// one-point halo.  Step s computes u'(s) on rows y0-1 .. y0+TY into ring B
-// (the halo columns x0-1 and x0+TX by the lanes of wave 0, which otherwise
-// only loads), and u''(s-2) from ring B on the tile's rows; one barrier per
-// step.
 std::string Translator::InteriorCond(const KernelInfo &k, const std::string &y,
                                      const std::string &z, int only) const {
   std::vector<std::string> c;
@@ -2055,6 +2047,15 @@ std::string Translator::SplitInterior(const KernelInfo &k, const std::string &y,
   return o.str();
 }
 
+// Two-sweep run kernel (temporal blocking) of a fusable kernel, its launcher
+// and the run-time check that a map pair may use it.  Block: BY waves, wave
+// r holds source row y0 - 2 + r; ring A (LDS) holds source planes with a
+// two-point x halo, ring B the intermediate planes with a one-point halo,
+// six slots each.  Interval s0 computes u'(s0) and u'(s0+1) on rows y0-1 ..
+// y0+TY into ring B (the halo columns x0-1 and x0+TX by the lanes of wave
+// 0, which otherwise only loads) and u''(s0-3), u''(s0-2) from ring B on the
+// tile's rows, then stores source planes s0+3 and s0+4: one barrier per two
+// planes.
 std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
   std::ostringstream o;
   const std::string &n = k.name;
