@@ -70,6 +70,7 @@ class Udt7:
         L.udt7_copyout.argtypes = [C.c_void_p, C.c_void_p]
         L.udt7_free.argtypes = [C.c_void_p]
         L.udt7_run.argtypes = [C.c_void_p, C.c_int] + [_F] * 7
+        L.udt7_whole_run.argtypes = [C.c_void_p, C.c_void_p] + [_F] * 7
         self.lib = L
 
     def new(self, shape, pq: np.ndarray):
@@ -90,6 +91,11 @@ class Udt7:
 
     def run(self, g, count: int, coef) -> None:
         self.lib.udt7_run(g, count, *[float(c) for c in coef])
+
+    def whole_run(self, g, g2, coef) -> None:
+        """One sweep g2 <- (p, 7-point(p)) over the interior, through
+        struct-valued gets and emit."""
+        self.lib.udt7_whole_run(g, g2, *[float(c) for c in coef])
 
 
 class Stencil2D:

@@ -4,8 +4,8 @@
  * tests/system_tests/test_cases/test_user-defined-type-7-pt.c:13-27 (read
  * member p of the seven points, emit member q of the centre), weighted as
  * the diffusion benchmark weighs its neighbours.  physisc-hip translates it
- * into udt7.hip.cpp; the run kernel is the generic one (one thread per
- * column), which this program measures.
+ * into udt7.hip.cpp; the translator splits the grid into its member planes
+ * and runs the 2.5D kernel over them, which this program measures.
  */
 #include "physis/physis.h"
 
@@ -24,6 +24,28 @@ static void udt7_kernel(const int x, const int y, const int z,
             cn * PSGridGet(g, x, y-1, z).p + cb * PSGridGet(g, x, y, z-1).p +
             ct * PSGridGet(g, x, y, z+1).p;
   PSGridEmitUtype(g.q, v);
+}
+
+/* The same sum over whole points (the pattern of
+ * test_user-defined-type-7-pt-periodic-complex.c:13-31: struct-valued gets,
+ * a struct-valued emit): the point (p, q') with q' the weighted 7-point sum
+ * of p, emitted into a second grid. */
+static void udt7_whole_kernel(const int x, const int y, const int z,
+                              PSGrid3DPoint g, PSGrid3DPoint g2, float ce,
+                              float cw, float cn, float cs, float ct, float cb,
+                              float cc) {
+  struct Point c = PSGridGet(g, x, y, z);
+  struct Point w = PSGridGet(g, x-1, y, z);
+  struct Point e = PSGridGet(g, x+1, y, z);
+  struct Point s = PSGridGet(g, x, y+1, z);
+  struct Point n = PSGridGet(g, x, y-1, z);
+  struct Point b = PSGridGet(g, x, y, z-1);
+  struct Point t = PSGridGet(g, x, y, z+1);
+  struct Point v;
+  v.p = c.p;
+  v.q = cc * c.p + cw * w.p + ce * e.p + cs * s.p + cn * n.p + cb * b.p +
+        ct * t.p;
+  PSGridEmit(g2, v);
 }
 
 PSGrid3DPoint udt7_new(int nx, int ny, int nz) {
@@ -45,4 +67,13 @@ void udt7_run(PSGrid3DPoint g, int count, float ce, float cw, float cn,
                                  PSGridDim(g, 1) - 1, 1, PSGridDim(g, 2) - 1);
   PSStencilRun(PSStencilMap(udt7_kernel, dom, g, ce, cw, cn, cs, ct, cb, cc),
                count);
+}
+
+/* One sweep g2 <- (p, 7-point(p)) over the interior, whole points. */
+void udt7_whole_run(PSGrid3DPoint g, PSGrid3DPoint g2, float ce, float cw,
+                    float cn, float cs, float ct, float cb, float cc) {
+  PSDomain3D dom = PSDomain3DNew(1, PSGridDim(g, 0) - 1, 1,
+                                 PSGridDim(g, 1) - 1, 1, PSGridDim(g, 2) - 1);
+  PSStencilRun(PSStencilMap(udt7_whole_kernel, dom, g, g2, ce, cw, cn, cs,
+                            ct, cb, cc));
 }
