@@ -216,6 +216,17 @@ inline View DevView(const __PSGrid *g) {
   return v;
 }
 
+// The domain of a whole 3-D grid (the tiling of a red-black pass, which
+// writes every point of its output).
+inline __PSDomain WholeDomain(const __PSGrid *g) {
+  __PSDomain d;
+  for (int i = 0; i < 3; ++i) {
+    d.min[i] = d.local_min[i] = 0;
+    d.max[i] = d.local_max[i] = g->dim[i];
+  }
+  return d;
+}
+
 // The rows view of a 2-D grid: the 3-D view nx x 1 x ny of the same
 // memory (equal linear offsets), in which a 2-D stencil runs on the 2.5D
 // path, marching y as z.

@@ -1024,7 +1024,40 @@ void Translator::AnalyzeFusion(KernelInfo &k) {
   if (!k.fast) return no("not a 2.5D run kernel");
   if (k.rows2d) return no("a 2-D map");
   if (k.index_rebound) return no("an index parameter is rebound");
-  if (k.used_rb) return no("used in red-black maps");
+  if (k.used_rb) {
+    // one pass per red-black iteration: the map reads only the grid it
+    // updates, at offsets of at most one point including the centre, and
+    // emits it once, unconditionally, at the top of its body
+    if (k.ret_type != "void") return no("a reducing red-black map");
+    int gi = -1;
+    for (int i = 0; i < (int)k.grids.size(); ++i) {
+      const GridParam &g = k.grids[i];
+      if (g.user) return no("a grid of user-defined points");
+      if (!g.read && !g.emitted) continue;
+      if (!(g.read && g.emitted) || gi >= 0 || g.irregular ||
+          g.mode != GridParam::kStaged || g.periodic)
+        return no("a red-black map reading other grids or periodic gets");
+      gi = i;
+    }
+    if (gi < 0) return no("a red-black map reading no grid");
+    const GridParam &g = k.grids[gi];
+    bool centre = false;
+    for (auto &o : g.offsets) centre = centre || (!o[0] && !o[1] && !o[2]);
+    if (!centre) return no("a red-black map not reading its centre");
+    if (g.hx > 1 || g.hy > 1 || g.zl < -1 || g.zr > 1)
+      return no("halo wider than one point");
+    if (CountEmits(*k.fn->body) != 1) return no("not exactly one emit");
+    bool top = false;
+    for (const auto &st : k.fn->body->kids)
+      if (st && st->kind == SK::kExpr && st->expr) {
+        const std::string *c = CalleeName(*Unparen(st->expr.get()));
+        top = top || (c && *c == "PSGridEmit");
+      }
+    if (!top) return no("the emit is conditional");
+    k.fuse_rb = true;
+    k.fuse_in = k.fuse_out = gi;
+    return;
+  }
   int in = -1, out = -1, direct = 0;
   for (int i = 0; i < (int)k.grids.size(); ++i) {
     const GridParam &g = k.grids[i];
@@ -1584,7 +1617,9 @@ std::string Translator::EmitContext(const KernelInfo &k) {
   if (k.used_rb) o << "//   red-black: " << k.rb_note << "\n";
   if (k.fast)
     o << "//   two-sweep fusion of map pairs: "
-      << (k.fusable ? std::string(k.fuse_domain
+      << (k.fuse_rb ? std::string("red-black: one pass per iteration (red, "
+                                  "then black) over a domain inside the grid")
+          : k.fusable ? std::string(k.fuse_domain
                                         ? "yes, over a domain inside the grid"
                                         : "yes")
                     : "no (" + k.fuse_why + ")") << "\n";
@@ -2070,6 +2105,25 @@ std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
   std::string fargs = "ctx";
   for (auto &a : sa) fargs += ", " + a;
   auto rows = RowsOf(gi);
+  // A red-black map (k.fuse_rb): u' is the red sweep, u'' the black one, in
+  // the REF order of one iteration; a point outside the map's domain or of
+  // the other colour keeps its value (the centre value of its row), and
+  // the pass tiles and writes the whole grid (gd).
+  const bool rb = k.fuse_rb;
+  int crow = -1;
+  for (size_t r = 0; r < rows.size(); ++r)
+    if (rows[r].dy == 0 && rows[r].dz == 0 && rows[r].xl <= 0 && rows[r].xr >= 0)
+      crow = (int)r;
+  if (rb && crow < 0) Fail(n, "red-black pass without the centre row");
+  const std::string TD = rb ? "gd" : "dom";
+  // the update test of a red-black pass at (x, y, z) for colour c
+  auto upd = [&](const std::string &x, const std::string &y,
+                 const std::string &z, int c) {
+    return "(" + x + " >= dom.local_min[0] && " + x + " < dom.local_max[0] && " +
+           y + " >= dom.local_min[1] && " + y + " < dom.local_max[1] && " + z +
+           " >= dom.local_min[2] && " + z + " < dom.local_max[2] && ((" + x +
+           " + " + y + " + " + z + " + " + std::to_string(c) + ") & 1) == 0)";
+  };
   // ctx fill of the lane's point i from row arrays <pre>_r<r>
   auto fill = [&](const std::string &pre, const std::string &ind) {
     std::ostringstream f;
@@ -2106,8 +2160,12 @@ std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
        "// planes s0 and s0 + 1 and u'' of s0 - 3 and s0 - 2 from rings of six\n"
        "// slots (source plane p in slot (p - zb + 2) % 6, u'(p) in slot\n"
        "// (p - zb + 1) % 6), then stores source planes s0 + 3 and s0 + 4.\n";
+  if (rb)
+    o << "// Red-black: one pass is one iteration (red sweep u', black sweep\n"
+         "// u''); dom is the map's domain, gd the whole grid the pass tiles.\n";
   o << "template <int VX, int BY>\n__global__ __launch_bounds__(64 * BY) void __PSStencilRun2_"
-    << n << "(\n    __PSDomain dom, " << V << " src, " << V << " dst, " << T
+    << n << "(\n    __PSDomain dom, " << (rb ? "__PSDomain gd, " : "") << V
+    << " src, " << V << " dst, " << T
     << " *__restrict__ mid" << (sp.empty() ? "" : ", " + Join(sp, ", "))
     << ", physis_hip::RunShape sh) {\n";
   o << "  constexpr int PD = 2, TX = 64 * VX, TY = BY - 4;\n"
@@ -2120,13 +2178,14 @@ std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
        "  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);\n"
        "  const physis_hip::TileCoord tc = physis_hip::TileOf(sh.tiles_x, sh.tiles_y);\n"
        "  const int x0 = (sh.x_first_tile + tc.bx) * TX;\n"
-       "  const int y0 = dom.local_min[1] + tc.by * TY;\n"
-       "  const int zb = dom.local_min[2] + tc.bz * sh.zchunk;\n"
-       "  const int ze = min(zb + sh.zchunk, dom.local_max[2]);\n"
+    << "  const int y0 = " << TD << ".local_min[1] + tc.by * TY;\n"
+    << "  const int zb = " << TD << ".local_min[2] + tc.bz * sh.zchunk;\n"
+    << "  const int ze = min(zb + sh.zchunk, " << TD << ".local_max[2]);\n"
        "  if (zb >= ze) return;  // uniform over the block\n"
        "  const int yrow = y0 - 2 + wave;\n"
        "  const bool mid_row = wave >= 1 && wave < BY - 1;\n"
-       "  const bool out_row = wave >= 2 && wave < BY - 2 && yrow < dom.local_max[1];\n"
+    << "  const bool out_row = wave >= 2 && wave < BY - 2 && yrow < " << TD
+    << ".local_max[1];\n"
        "  const int xs = x0 + lane * VX;\n"
        "  const int nx = src.dim[0], nxy = src.dim[0] * src.dim[1];\n"
        "  const int n = nxy * src.dim[2];\n"
@@ -2166,8 +2225,12 @@ std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
     b << "#pragma unroll\n      for (int i = 0; i < VX; ++i) {\n";
     b << fill("a", "        ");
     b << "        ctx." << O << "_e = false;\n"
-      << "        " << n << "(xs + i, yrow, s, " << fargs << ");\n"
-      << "        uo[i] = ctx." << O << "_v;\n      }\n";
+      << "        " << n << "(xs + i, yrow, s, " << fargs << ");\n";
+    if (rb)
+      b << "        uo[i] = " << upd("(xs + i)", "yrow", "s", 0) << " ? ctx." << O
+        << "_v : a_r" << crow << "[i + " << -rows[crow].xl << "];\n      }\n";
+    else
+      b << "        uo[i] = ctx." << O << "_v;\n      }\n";
     o << SplitInterior(k, "yrow", "s", b.str(), "      ", k.fuse_in);
   }
   o << "      SB::WriteRow(b_lds + bo * SB::kPlane, wave, lane, uo);\n";
@@ -2187,9 +2250,14 @@ std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
       << ") * SA::kRow + SA::kPad + px + " << off[0] << "];\n";
   }
   o << "      ctx." << O << "_e = false;\n"
-    << "      " << n << "(x0 + px, y0 - 2 + hr, s, " << fargs << ");\n"
-    << "      b_lds[bo * SB::kPlane + hr * SB::kRow + SB::kPad + px] = ctx."
-    << O << "_v;\n    }\n  };\n";
+    << "      " << n << "(x0 + px, y0 - 2 + hr, s, " << fargs << ");\n";
+  if (rb)
+    o << "      b_lds[bo * SB::kPlane + hr * SB::kRow + SB::kPad + px] = "
+      << upd("(x0 + px)", "(y0 - 2 + hr)", "s", 0) << " ? ctx." << O
+      << "_v : a_lds[sc * SA::kPlane + hr * SA::kRow + SA::kPad + px];\n    }\n  };\n";
+  else
+    o << "      b_lds[bo * SB::kPlane + hr * SB::kRow + SB::kPad + px] = ctx."
+      << O << "_v;\n    }\n  };\n";
   // u''(z) of the wave's row, u'(z - 1), u'(z), u'(z + 1) in B slots sm,
   // sc, sp
   o << "  auto __udouble = [&](const int z, const int sm, const int sc, "
@@ -2204,10 +2272,14 @@ std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
     b << "#pragma unroll\n      for (int i = 0; i < VX; ++i) {\n";
     b << fill("b", "        ");
     b << "        ctx." << O << "_e = false;\n"
-      << "        " << n << "(xs + i, yrow, z, " << fargs << ");\n"
-      << "        uu[i] = ctx." << O << "_v;\n"
-      << "        m |= (xs + i >= dom.local_min[0] && xs + i < dom.local_max[0]) "
-         "? 1u << i : 0u;\n      }\n";
+      << "        " << n << "(xs + i, yrow, z, " << fargs << ");\n";
+    if (rb)
+      b << "        uu[i] = " << upd("(xs + i)", "yrow", "z", 1) << " ? ctx." << O
+        << "_v : b_r" << crow << "[i + " << -rows[crow].xl << "];\n";
+    else
+      b << "        uu[i] = ctx." << O << "_v;\n";
+    b << "        m |= (xs + i >= " << TD << ".local_min[0] && xs + i < " << TD
+      << ".local_max[0]) ? 1u << i : 0u;\n      }\n";
     o << SplitInterior(k, "yrow", "z", b.str(), "      ", k.fuse_in);
   }
   o << "      physis_hip::StoreMasked<" << T
@@ -2237,22 +2309,43 @@ std::string Translator::EmitFusedRunKernel(const KernelInfo &k) {
   o << "static constexpr int __PS2VX_" << n << " = " << vx << ", __PS2BY_"
     << n << " = " << by << ";\n";
   std::vector<std::string> largs = {"s.dom", "src", "dst", "mid"};
+  if (rb) largs = {"s.dom", "gd", "src", "dst", "mid"};
   for (auto &a : sa) largs.push_back("s." + a);
   largs.push_back("sh");
   o << "static void __PSLaunch2_" << n << "(const struct __PSStencil_" << n
     << " &s, const " << V << " &src, const " << V << " &dst, " << T
     << " *mid, hipStream_t stream) {\n"
     << "  constexpr int VX = __PS2VX_" << n << ", BY = __PS2BY_" << n << ";\n"
+    << (rb ? "  const __PSDomain gd = physis_hip::WholeDomain(s." + I + ");\n" : "")
     << "  static int per_cu = physis_hip::BlocksPerCU(\n"
        "      (const void *)__PSStencilRun2_" << n << "<VX, BY>, 64 * BY);\n"
-    << "  const physis_hip::RunShape sh = physis_hip::PlanRun(s.dom, 64 * VX, "
-       "BY - 4, per_cu,\n"
+    << "  const physis_hip::RunShape sh = physis_hip::PlanRun("
+    << (rb ? "gd" : "s.dom") << ", 64 * VX, BY - 4, per_cu,\n"
        "      physis_hip::WindowChunkLimit({s." << I << "}, 3));\n"
     << "  if (sh.blocks == 0) return;\n"
     << "  hipLaunchKernelGGL((__PSStencilRun2_" << n << "<VX, BY>), "
        "dim3(sh.blocks), dim3(64 * BY), 0, stream, "
     << Join(largs, ", ") << ");\n}\n\n";
   // run-time check
+  if (rb) {
+    // one map of both colours over a domain at least one point inside the
+    // grid (so no point of the domain reads past a face, and the halo
+    // points the pass computes outside the grid are never updated)
+    o << "// The red-black map s0 may run as one pass per iteration: both "
+         "colours, a domain\n// strictly inside a grid whose x extent is a "
+         "multiple of 4.\n";
+    o << "static bool __PSFusableRB_" << n << "(const struct __PSStencil_" << n
+      << " &s0) {\n"
+      << "  static const bool off = getenv(\"PHYSIS_HIP_NO_FUSE\") != NULL;\n"
+      << "  const __PSGrid *a = s0." << I << ";\n"
+      << "  if (off || !a || s0.__rb != 2 || a->num_dims != 3 || a->dim[0] % 4) "
+         "return false;\n"
+      << "  for (int d = 0; d < 3; ++d)\n"
+         "    if (s0.dom.local_min[d] < 1 || s0.dom.local_max[d] > a->dim[d] - 1 ||\n"
+         "        s0.dom.local_min[d] >= s0.dom.local_max[d]) return false;\n"
+      << "  return true;\n}\n\n";
+    return o.str();
+  }
   o << "// The pair (s0, s1) is " << n << "(a -> b), " << n
     << "(b -> a) over the whole grid with the\n// same scalars: it may run as "
        "two-sweep passes.\n";
@@ -2464,7 +2557,7 @@ std::string Translator::EmitKernel(KernelInfo &k) {
   s += EmitDeviceFunction(k);
   s += k.fast ? EmitFastRunKernel(k) : EmitGenericRunKernel(k);
   s += EmitLauncher(k);
-  if (k.fusable)
+  if (k.fusable || k.fuse_rb)
     s += k.fuse_domain ? EmitFusedDomainKernel(k) : EmitFusedRunKernel(k);
   return s;
 }
@@ -2893,7 +2986,32 @@ std::string Translator::EmitRunFunction(const RunSite &r) {
   const bool fuse = r.kernels.size() == 2 && r.kernels[0] == r.kernels[1] &&
                     kernels_.at(r.kernels[0]).fusable;
   if (fuse) o << EmitFusedRun(kernels_.at(r.kernels[0]));
-  o << "  for (int i = " << (fuse ? "i0" : "0") << "; i < iter; ++i) {\n";
+  // a red-black map of both colours: one pass per iteration, ping-ponging
+  // between the grid and a scratch copy (every pass writes the whole grid);
+  // an odd count ends with a copy back
+  const bool fuse_rb = r.kernels.size() == 1 && kernels_.at(r.kernels[0]).fuse_rb;
+  if (fuse_rb) {
+    const KernelInfo &k = kernels_.at(r.kernels[0]);
+    const GridParam &g = k.grids[k.fuse_in];
+    const std::string V = g.dev_view, T = g.elem;
+    o << "  int i0 = 0;\n"
+      << "  if (iter >= 1 && __PSFusableRB_" << k.name << "(s0)) {\n"
+      << "    physis_hip::CheckGrid3D(s0." << g.name << ", " << g.ps_type << ");\n"
+      << "    const " << V << " va = physis_hip::DevView<" << V << ">(s0." << g.name
+      << ");\n"
+      << "    const size_t bytes = (size_t)s0." << g.name << "->num_elms * sizeof("
+      << T << ");\n"
+      << "    " << V << " vt = va;\n"
+      << "    vt.p = (" << T << " *)__PSHipScratch(8, bytes);\n"
+      << "    " << V << " src = va;\n"
+      << "    for (int j = 0; j < iter; ++j) {\n"
+      << "      const " << V << " dst = (j % 2 == 0) ? vt : va;\n"
+      << "      __PSLaunch2_" << k.name << "(s0, src, dst, NULL, stream);\n"
+      << "      src = dst;\n    }\n"
+      << "    if (iter % 2) physis_hip::CopyDevice(va.p, vt.p, bytes, stream);\n"
+      << "    i0 = iter;\n  }\n";
+  }
+  o << "  for (int i = " << (fuse || fuse_rb ? "i0" : "0") << "; i < iter; ++i) {\n";
   for (size_t i = 0; i < r.kernels.size(); ++i) {
     const KernelInfo &k = kernels_.at(r.kernels[i]);
     o << "    __PSRun_" << k.name << "(s" << i << ", stream, "

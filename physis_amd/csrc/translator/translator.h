@@ -99,6 +99,8 @@ struct KernelInfo {
   // the pair fuses over a domain strictly inside the grid (points outside
   // it keep their values), rather than over the whole grid
   bool fuse_domain = false;
+  // a red-black map run as one pass per iteration (red sweep, then black)
+  bool fuse_rb = false;
   int fuse_in = -1, fuse_out = -1;  // grid indices of the read and emitted grids
   std::string fuse_why;             // why not, for the analysis comment
   // the body compares index parameter y / z (axis 1 / 2) with something:

@@ -163,7 +163,20 @@ void run(PSGrid3DFloat g) {
     assert "__PSStencilMapRedBlack_k(d, g)" in code
     assert "__PSStencilMapBlack_k(d, g)" in code
     assert "reads only the other colour" in code and not err
-    assert "no (used in red-black maps)" in code
+    assert "no (a red-black map not reading its centre)" in code
+    # with its centre read, a map of both colours run alone becomes one
+    # pass per iteration (red sweep, then black), guarded at run time
+    one = src.replace("PSGridGet(g, x+1, y, z) +",
+                      "PSGridGet(g, x, y, z) * 0.5f + PSGridGet(g, x+1, y, z) +")
+    one = one.replace(", PSStencilMapBlack(k, d, g));", ", 3);")
+    rc, code, err = translate(None, one)
+    assert rc == 0, err
+    assert "red-black: one pass per iteration (red, then black)" in code
+    assert "if (iter >= 1 && __PSFusableRB_k(s0)) {" in code
+    assert "if (iter % 2) physis_hip::CopyDevice(va.p, vt.p, bytes, stream);" in code
+    assert "const __PSDomain gd = physis_hip::WholeDomain(s.g);" in code
+    assert "(((xs + i) + yrow + s + 0) & 1) == 0) ? ctx.g_v : a_r" in code
+    assert "(((xs + i) + yrow + z + 1) & 1) == 0) ? ctx.g_v : b_r" in code
     # a get of the point's own colour is flagged and keeps the generic
     # kernel, which skips the other colour's points
     rc, code, err = translate(None, src.replace("x+1, y, z", "x+1, y+1, z"))
