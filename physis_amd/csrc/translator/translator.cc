@@ -498,10 +498,11 @@ struct UserSplit {
   // A whole point is read or written member by member: every member must
   // be a scalar, and the expressions that are repeated per member (the
   // indices of a get, the value of an emit) must be free of side effects.
-  bool Whole() {
+  bool Whole(const std::vector<bool> *use = nullptr) {
     for (auto &m : members)
       if (!m.extents.empty() || !IsScalarType(m.type)) return false;
-    for (size_t i = 0; i < members.size(); ++i) Use((int)i, 0);
+    for (size_t i = 0; i < members.size(); ++i)
+      if (!use || (*use)[i]) Use((int)i, 0);
     return true;
   }
   static bool Pure(const Expr &e) {
@@ -612,9 +613,82 @@ struct UserSplit {
     for (const auto &kid : e.kids)
       if (kid) Check(*kid);
   }
+  // A local point initialised by a whole point read, `struct T v =
+  // PSGridGet(g, i...)`, that is used only through its members reads just
+  // those members' planes (the others are set to 0 and never read).
+  // points: such variables and the members they use; a name declared twice
+  // or used whole (passed, assigned, its address taken) uses every member.
+  std::map<std::string, std::vector<bool>> points;
+  bool IsWholeGet(const Declarator &d) const {
+    const Expr *get = d.init ? GetOf(d.init.get()) : nullptr;
+    return get && get == Unp(d.init.get()) && IsG(get->kids[1].get()) &&
+           d.prefix.empty() && d.suffix.empty() && !d.ctor_init;
+  }
+  void ScanDecls(const Stmt &st, std::map<std::string, int> &seen) {
+    for (const auto &d : st.decls) {
+      ++seen[d.name];
+      if (IsWholeGet(d)) points[d.name].assign(members.size(), false);
+    }
+    for (const auto &kid : st.kids)
+      if (kid) ScanDecls(*kid, seen);
+    if (st.init) ScanDecls(*st.init, seen);
+  }
+  void ScanUses(const Expr &e, const Expr *parent) {
+    if (e.kind == EK::kIdent && points.count(e.text)) {
+      std::vector<bool> &u = points[e.text];
+      int m = -1;
+      if (parent && parent->kind == EK::kMember && parent->op == "." &&
+          parent->kids[0].get() == &e)
+        for (size_t i = 0; i < members.size(); ++i)
+          if (members[i].name == parent->text) m = (int)i;
+      if (m >= 0)
+        u[m] = true;
+      else
+        u.assign(members.size(), true);
+    }
+    for (const auto &kid : e.kids)
+      if (kid) ScanUses(*kid, &e);
+  }
+  void ScanUses(const Stmt &st) {
+    for (const auto &d : st.decls)
+      if (d.init) ScanUses(*d.init, nullptr);
+    for (const Expr *e : {st.expr.get(), st.cond.get(), st.step.get()})
+      if (e) ScanUses(*e, nullptr);
+    if (st.init) ScanUses(*st.init);
+    for (const auto &kid : st.kids)
+      if (kid) ScanUses(*kid);
+  }
+  void ScanPoints(const Stmt &body) {
+    std::map<std::string, int> seen;
+    ScanDecls(body, seen);
+    for (auto &it : seen)
+      if (it.second > 1) points.erase(it.first);
+    ScanUses(body);
+  }
+  // the members a whole point read initialising d reads
+  std::vector<bool> PointUse(const Declarator &d) const {
+    auto it = points.find(d.name);
+    if (!IsWholeGet(d) || it == points.end())
+      return std::vector<bool>(members.size(), true);
+    return it->second;
+  }
+
   void Check(const Stmt &st) {
     for (const auto &d : st.decls) {
       if (d.name == g) ok = false;  // shadowed
+      if (d.init && IsWholeGet(d)) {
+        // a whole point read into a local point: the members it uses
+        const Expr *get = GetOf(d.init.get());
+        for (size_t i = 2; i < get->kids.size(); ++i)
+          if (!Pure(*get->kids[i])) ok = false;
+        const std::vector<bool> use = PointUse(d);
+        if (!ok || !Whole(&use)) {
+          ok = false;
+          return;
+        }
+        for (size_t i = 2; i < get->kids.size(); ++i) Check(*get->kids[i]);
+        continue;
+      }
       if (d.init) Check(*d.init);
     }
     for (const Expr *e : {st.expr.get(), st.cond.get(), st.step.get()})
@@ -697,8 +771,29 @@ struct UserSplit {
       if (kid) Rewrite(kid);
   }
   void Rewrite(Stmt &st) {
-    for (auto &d : st.decls)
+    for (auto &d : st.decls) {
+      if (d.init && IsWholeGet(d)) {
+        // __PSUdtMake_<type>(PSGridGet(g__m, i...) or 0 per member)
+        const std::vector<bool> use = PointUse(d);
+        const Expr *get = GetOf(d.init.get());
+        ExprP make = Call("__PSUdtMake_" + uname);
+        for (size_t m = 0; m < members.size(); ++m) {
+          if (!use[m]) {
+            auto zero = std::make_unique<Expr>(EK::kLiteral);
+            zero->text = "0";
+            make->kids.push_back(std::move(zero));
+            continue;
+          }
+          ExprP one = Clone(*get);
+          one->kids[1] = Ident(used[Use((int)m, 0)].name);
+          for (size_t i = 2; i < one->kids.size(); ++i) Rewrite(one->kids[i]);
+          make->kids.push_back(std::move(one));
+        }
+        d.init = std::move(make);
+        continue;
+      }
       if (d.init) Rewrite(d.init);
+    }
     for (ExprP *e : {&st.expr, &st.cond, &st.step})
       if (*e) Rewrite(*e);
     if (st.init) Rewrite(*st.init);
@@ -755,6 +850,7 @@ void Translator::AnalyzeKernel(KernelInfo &k) {
     for (size_t gi = 0; gi < nreal; ++gi) {
       if (!k.grids[gi].user || k.grids[gi].dims != 3) continue;
       UserSplit us(k.grids[gi].name, k.grids[gi].uname, *k.grids[gi].members);
+      us.ScanPoints(*f.body);
       us.Check(*f.body);
       if (!us.ok || us.used.empty()) continue;
       us.Rewrite(*const_cast<Function &>(f).body);

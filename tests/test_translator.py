@@ -313,8 +313,11 @@ def test_user_point_grids_split_into_member_grids():
     rc, code, err = translate(None, SPLIT_PROG)
     assert rc == 0, err
     assert "Stencil analysis of whole (2.5D run kernel)" in code
-    assert "g1__r  staged x[-1,1] y[-1,1] z[-1,0] non-periodic" in code
+    # a local point initialised by a whole read reads only the members it
+    # uses: b uses b.i alone, so g1__r is not read at (0, 0, -1)
+    assert "g1__r  staged x[-1,1] y[-1,1] z[0,0] non-periodic" in code
     assert "g1__i  staged x[-1,1] y[-0,0] z[-1,0] non-periodic" in code
+    assert "struct Cplx b = __PSUdtMake_Cplx(0, __ctx.g1__i[1]);" in code
     # a whole point read member by member, a whole point emitted likewise
     assert "__PSUdtMake_Cplx(__ctx.g1__r[" in code
     assert "(__ctx.g2__r_v = ((v).r), __ctx.g2__r_e = true)" in code
