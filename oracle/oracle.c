@@ -263,6 +263,25 @@ void oracle_redblack7_run(float *g, int nx, int ny, int nz, int count,
 #undef G
 }
 
+/* ------------------------------- 7-point map with 1-D axis coefficients */
+/* One sweep of physis_amd/programs/mixed7.c over [1, n-1)^3, the kernel of
+ * test_mixed-dim2.c:12-23 in its operand order: v = g + g[x-1] cx[x-1] +
+ * g[x+1] cx[x+1] + g[y-1] cy[y-1] + g[y+1] cy[y+1] + g[z-1] cz[z-1] +
+ * g[z+1] cz[z+1] (points outside keep o). */
+void oracle_mixed7_sweep(const float *g, float *o, int nx, int ny, int nz,
+                         const float *cx, const float *cy, const float *cz) {
+#define G(a, b, c) g[OFF(a, b, c, nx, ny)]
+  for (int z = 1; z < nz - 1; ++z)
+    for (int y = 1; y < ny - 1; ++y)
+      for (int x = 1; x < nx - 1; ++x)
+        o[OFF(x, y, z, nx, ny)] =
+            G(x, y, z) + G(x - 1, y, z) * cx[x - 1] +
+            G(x + 1, y, z) * cx[x + 1] + G(x, y - 1, z) * cy[y - 1] +
+            G(x, y + 1, z) * cy[y + 1] + G(x, y, z - 1) * cz[z - 1] +
+            G(x, y, z + 1) * cz[z + 1];
+#undef G
+}
+
 /* ------------------------------------------------- 2-D 5- and 9-point maps */
 /* sweeps of physis_amd/programs/stencil2d.c, alternating a -> b and b -> a
  * (the first from a), each over [1, nx-1) x [1, ny-1) in the REFERENCE

@@ -103,6 +103,11 @@ int oracle_himeno_run_omp(float *p0, float *p1, const float *const *coef,
 void oracle_redblack7_run(float *g, int nx, int ny, int nz, int count,
                           const float *coef);
 
+/* ---- 7-point map weighted by 1-D axis coefficient grids
+ * (physis_amd/programs/mixed7.c): one sweep o <- kernel(g) over [1, n-1)^3. */
+void oracle_mixed7_sweep(const float *g, float *o, int nx, int ny, int nz,
+                         const float *cx, const float *cy, const float *cz);
+
 /* ---- 2-D 5- and 9-point maps (physis_amd/programs/stencil2d.c): `sweeps`
  * sweeps alternating a -> b, b -> a over [1, nx-1) x [1, ny-1); out = (sum of
  * the points in the reference tests' order) * w. */

@@ -62,6 +62,7 @@ PROGRAM_SOURCES = [os.path.join(PROGRAMS, "diffusion7.c"),
                    os.path.join(PROGRAMS, "redblack7.c"),
                    os.path.join(PROGRAMS, "udt7.c"),
                    os.path.join(PROGRAMS, "stencil2d.c"),
+                   os.path.join(PROGRAMS, "mixed7.c"),
                    os.path.join(ROOT, "tests", "programs", "systests.c")]
 
 

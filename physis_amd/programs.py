@@ -118,6 +118,22 @@ class Stencil2D:
         self.lib.stencil2d_sweep(a.handle, b.handle, points, float(w))
 
 
+class Mixed7:
+    """physis_amd/programs/mixed7.c: a 3-D 7-point map weighted by 1-D
+    coefficient grids along each axis."""
+
+    def __init__(self):
+        L = rt.load_program("mixed7")
+        G = rt.PSGridPtr
+        L.mixed7_run.argtypes = [G] * 5
+        self.lib = L
+
+    def run(self, g, g2, cx, cy, cz) -> None:
+        """One sweep g2 <- kernel(g) over the interior."""
+        self.lib.mixed7_run(g.handle, g2.handle, cx.handle, cy.handle,
+                            cz.handle)
+
+
 class Himeno:
     """physis_amd/programs/himeno.c: 19-point Himeno Jacobi sweeps."""
 

@@ -75,6 +75,8 @@ def lib() -> C.CDLL:
                                         FP]
         L.oracle_stencil2d_run.argtypes = [FP, FP, C.c_int, C.c_int, C.c_int,
                                            C.c_float, C.c_int]
+        L.oracle_mixed7_sweep.argtypes = [FP, FP, C.c_int, C.c_int, C.c_int,
+                                          FP, FP, FP]
         L.oracle_test_27pt_int.argtypes = [IP, IP, C.c_int]
         L.oracle_diffusion_coefs.argtypes = [C.c_int, C.c_int, C.c_int, FP]
         L.oracle_diffusion_init.argtypes = [FP, C.c_int, C.c_int, C.c_int,

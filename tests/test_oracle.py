@@ -396,3 +396,35 @@ def test_stencil2d_oracle_linear_field(points):
     assert np.array_equal(b.reshape(ny, nx)[1:-1, 1:-1],
                           (points * f.reshape(ny, nx)[1:-1, 1:-1] *
                            np.float32(0.5)).astype(np.float32))
+
+
+def test_mixed7_oracle_matches_reference_output():
+    """oracle_mixed7_sweep is the reference's test_mixed-dim2 (g = i, output
+    grid initialised to g, coefficients 1 + (i % 2) on every axis, one map
+    over [1, N-1)^3; integer sums below 2^24 are exact): its "%f" dump
+    hashes to the reference oracle's output."""
+    n = 32
+    g = np.arange(n ** 3, dtype=np.float32)
+    o = g.copy()
+    c = (1 + np.arange(n) % 2).astype(np.float32)
+    lib().oracle_mixed7_sweep(g, o, n, n, n, c, c, c)
+    dump = "".join("%f\n" % v for v in o.astype(np.float64).tolist()).encode()
+    assert sha256(dump) == GOLD["system_tests"]["test_mixed-dim2"]["sha256"]
+
+
+def test_mixed7_oracle_axis_weights():
+    """Distinct weights per axis on a non-cubic grid: with g = 1 the interior
+    equals 1 + the sum of the six neighbouring coefficients, so a swapped
+    axis or coefficient index shows."""
+    nx, ny, nz = 9, 7, 5
+    g = np.ones(nx * ny * nz, np.float32)
+    o = np.full_like(g, -1.0)
+    cx = np.arange(nx, dtype=np.float32)
+    cy = 100 * np.arange(ny, dtype=np.float32)
+    cz = 10000 * np.arange(nz, dtype=np.float32)
+    lib().oracle_mixed7_sweep(g, o, nx, ny, nz, cx, cy, cz)
+    o = o.reshape(nz, ny, nx)
+    z, y, x = 2, 3, 4
+    assert o[z, y, x] == 1 + cx[x - 1] + cx[x + 1] + cy[y - 1] + cy[y + 1] + \
+        cz[z - 1] + cz[z + 1]
+    assert o[0, 0, 0] == -1.0 and o[z, y, 0] == -1.0

@@ -11,7 +11,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PHYSISC = os.path.join(ROOT, "physis_amd", "bin", "physisc-hip")
 SOURCES = [os.path.join(ROOT, "physis_amd", "programs", f)
            for f in ("diffusion7.c", "himeno.c", "redblack7.c", "udt7.c",
-                     "stencil2d.c")] + \
+                     "stencil2d.c", "mixed7.c")] + \
           [os.path.join(ROOT, "tests", "programs", "systests.c")]
 
 
