@@ -17,9 +17,13 @@ def rb(ps):
     return RedBlack7()
 
 
+# x extents that are multiples of 4 run one pass per iteration (odd counts
+# end with a copy back), the others two colour sweeps
 @pytest.mark.parametrize("shape,count", [((130, 66, 40), 3), ((36, 9, 7), 2),
                                          ((64, 64, 64), 1), ((33, 17, 20), 2),
-                                         ((512, 512, 512), 2)])
+                                         ((128, 96, 80), 3),
+                                         ((512, 512, 512), 2),
+                                         ((512, 512, 512), 3)])
 def test_redblack_matches_oracle(ps, rb, shape, count):
     print(f"red-black {shape} x{count}", flush=True)
     a = diffusion_input(*shape, seed=9)
