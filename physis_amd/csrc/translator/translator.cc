@@ -2597,6 +2597,27 @@ std::string Translator::EmitLauncher(const KernelInfo &k) {
     o << "static int __PSRun_" << n << "(const struct __PSStencil_" << n
       << " &s, hipStream_t stream, void *red) {\n";
     for (auto &c : checks) o << "  " << c << ";\n";
+    // A periodic neighbour across an odd extent has the point's own colour
+    // (x = n-1 and x = 0 share a parity): REF's sequential sweep then reads
+    // a value it already updated, a colour sweep in parallel the old one.
+    if (k.used_rb) {
+      std::vector<std::string> odd;
+      for (auto &g : k.grids) {
+        if (!g.read || !g.emitted || g.dims != 3) continue;
+        for (size_t i = 0; i < g.offsets.size(); ++i)
+          for (int a = 0; a < 3; ++a)
+            if (g.offset_periodic[i] && g.offsets[i][a])
+              add(odd, HostGrid(k, g) + "->dim[" + std::to_string(a) + "] % 2");
+      }
+      if (!odd.empty())
+        o << "  if (" << Join(odd, " || ") << ") {\n"
+          << "    static bool __warned = false;\n"
+          << "    if (!__warned) std::fprintf(stderr, \"[physis-hip] warning: "
+          << n << ": a periodic neighbour across an odd extent has the "
+             "point's own colour; colours are updated in parallel, not in "
+             "the REFERENCE order\\n\");\n"
+          << "    __warned = true;\n  }\n";
+    }
     for (auto &c : emit_checks) o << "  " << c << ";\n";
     o << "  return (" << vec << ") ? __PSLaunch_" << n
       << "<4>(s, stream, red) : __PSLaunch_" << n << "<1>(s, stream, red);\n}\n\n";
